@@ -1,0 +1,173 @@
+"""Benchmark of the MI355X Wilson even-odd Dslash + CG path.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cg-solves N] [--no-cpu]
+
+Workload: the plain-method GPU configuration -- 32^4 lattice, FP64, one right-hand side,
+links exp(i 0.3 H) (seed 3), kappa 0.125, Gaussian source (seed 4), antiperiodic in t.
+A step is one half-lattice hopping application D_oe (out on V/2 sites), the unit of work of
+the Dslash metric.  After the Dslash, CG on M^dag M is run to 1e-10 (relative, with the FP64
+true residual recomputed) for --cg-solves solves: solves/s and p50/p99 latency.
+
+Prints ONE JSON line.  `value` is the Dslash throughput in algorithmic GB/s: 1536 bytes per
+output site in FP64 (8 links x 144 B + one input and one output spinor x 192 B), i.e. each
+link and spinor counted once per application.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0            # MI355X HBM3E, 8 TB/s
+BYTES_PER_SITE_F64 = 1536        # 8*144 + 192 + 192
+FLOPS_PER_SITE = 1320
+DIMS = (32, 32, 32, 32)
+KAPPA = 0.125
+METRIC = "Dslash GB/s & %HBM roofline; CG solves/s + p99 latency @1e-10 true residual"
+PMC_FILE = os.path.join(ROOT, "profiles", "dslash_f64_32x32x32x32_pmc.json")
+
+
+def cpu_baseline(U, psi, seconds=15.0):
+    """The oracle's HILA-structured hopping term (projection pass + per-direction
+    accumulation, wilson.h:53-95) on the host cores, on a bounded number of applications."""
+    from oracle.oracle import ODD, Oracle
+    orc = Oracle(DIMS)
+    orc.hop_set(U, KAPPA, psi, ODD, 1)  # warm-up (page-in, thread pool)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        orc.hop_set(U, KAPPA, psi, ODD, 1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 200:
+            break
+    vh = orc.Vh
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": round(BYTES_PER_SITE_F64 * vh * n / el / 1e9, 3),
+        "unit": "GB/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{n} half-lattice hopping applications (Dirac_Wilson_hop_set restated, "
+                  f"HILA pass structure) at 32^4 FP64, {el:.1f} s, OpenMP {cores} threads",
+        "ms_per_apply": round(1e3 * el / n, 3),
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cg-solves", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is a design note only)"}))
+        sys.exit(2)
+
+    from hila_amd import wilson as W
+
+    t_setup = time.perf_counter()
+    U = W.gen_gauge(DIMS, "smooth", 0.3, 3)
+    psi = W.gen_spinor(DIMS, 4, W.EVEN)
+    lat = W.Lattice(DIMS, antiperiodic_t=True)
+    lat.upload_gauge(U)
+    D = W.Dirac_Wilson_evenodd(KAPPA, lat)
+    fin = W.Field(lat, W.EVEN).upload(psi)
+    fodd = W.Field(lat, W.ODD)
+    vh = lat.Vh
+    print(f"# setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+
+    # --- Dslash: W warm-up, K timed steps, synchronised on both sides ---
+    for _ in range(a.warmup):
+        D.hop_set(fin, fodd, 1)
+    lat.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        D.hop_set(fin, fodd, 1)
+    lat.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / a.steps
+    # the kernel's own average duration, HIP events on the library's stream
+    ms_kernel = D.time(fin, fodd, "hop", 1, warmup=5, iters=max(a.steps, 100))
+    ms_M = D.time(fin, W.Field(lat, W.EVEN), "M", 1, warmup=3, iters=50)
+    alg_bytes = BYTES_PER_SITE_F64 * vh
+    gbs_wall = alg_bytes / (ms_per_step * 1e-3) / 1e9
+    gbs_kernel = alg_bytes / (ms_kernel * 1e-3) / 1e9
+    print(f"# dslash {ms_kernel * 1e3:.1f} us/launch {gbs_kernel:.0f} GB/s", file=sys.stderr,
+          flush=True)
+
+    # --- CG solves to 1e-10 ---
+    cg = W.CG(D, 1e-10, 10000)
+    fx = W.Field(lat, W.EVEN)
+    lat_ms, iters, tres = [], [], []
+    for i in range(a.cg_solves):
+        fx.zero()
+        t = time.perf_counter()
+        st = cg.apply(fin, fx)[0]
+        lat_ms.append(1e3 * (time.perf_counter() - t))
+        iters.append(st["iterations"])
+        tres.append(st["true_resid"])
+        print(f"# cg solve {i}: {lat_ms[-1]:.1f} ms, {st['iterations']} it, "
+              f"true resid {st['true_resid']:.2e}", file=sys.stderr, flush=True)
+    cg_out = None
+    if lat_ms:
+        arr = np.array(lat_ms)
+        cg_out = {
+            "solves_per_s": round(1e3 / arr.mean(), 4),
+            "p50_ms": round(float(np.percentile(arr, 50)), 2),
+            "p99_ms": round(float(np.percentile(arr, 99)), 2),
+            "iterations": int(np.median(iters)),
+            "max_true_resid": float(max(tres)),
+            "ms_per_iteration": round(float(arr.mean()) / float(np.mean(iters)), 4),
+            "precision": "f64",
+            "solves": len(lat_ms),
+        }
+
+    traffic = None
+    if os.path.exists(PMC_FILE):
+        with open(PMC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": round(gbs_wall, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded exp(i*0.3*H) links, Gaussian source)",
+        "config": {"workload": "wilson_evenodd_dslash_32x32x32x32_f64_1rhs",
+                   "lattice": list(DIMS), "kappa": KAPPA, "bc_t": "antiperiodic",
+                   "step": "one half-lattice hopping application D_oe (V/2 output sites)"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs_kernel, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs_kernel / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": "k_dslash<double,+1,EPI_SET>",
+                     "kernel_us": round(ms_kernel * 1e3, 2),
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "gflops": round(FLOPS_PER_SITE * vh / (ms_kernel * 1e-3) / 1e9, 1),
+        "meo_us": round(ms_M * 1e3, 2),
+        "cg": cg_out,
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(U, np.ascontiguousarray(psi))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

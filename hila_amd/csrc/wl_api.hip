@@ -1,0 +1,540 @@
+// C ABI of libhila_wilson.so: contexts, device fields, the even-odd Wilson operator and CG.
+// Declarations and conventions: include/wilson_mi355x.h.
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "wl_internal.h"
+
+using namespace wl;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(WL_E_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+// Make the context's device current for the calling thread.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+int ensure_staging(wl_ctx *c, size_t bytes) {
+    if (c->staging_bytes >= bytes) return WL_OK;
+    if (c->staging) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(c->staging));
+        c->staging = nullptr;
+        c->staging_bytes = 0;
+    }
+    if (hipMalloc(&c->staging, bytes) != hipSuccess)
+        return fail(WL_E_OOM, "staging buffer of %zu bytes", bytes);
+    c->staging_bytes = bytes;
+    return WL_OK;
+}
+
+int field_new(wl_ctx *c, int parity, int prec, int nrhs, wl_field **out) {
+    wl_field *f = new (std::nothrow) wl_field;
+    if (!f) return fail(WL_E_OOM, "host allocation");
+    f->ctx = c;
+    f->parity = parity;
+    f->prec = prec;
+    f->nrhs = nrhs;
+    f->rhs_elems = (size_t)12 * c->g.stride;
+    f->bytes = f->rhs_elems * nrhs * prec_bytes(prec);
+    if (hipMalloc(&f->data, f->bytes) != hipSuccess) {
+        const size_t want = f->bytes;
+        delete f;
+        return fail(WL_E_OOM, "device field of %zu bytes", want);
+    }
+    if (hipMemsetAsync(f->data, 0, f->bytes, c->stream) != hipSuccess) {
+        (void)hipFree(f->data);
+        delete f;
+        return fail(WL_E_DEVICE, "hipMemsetAsync of a new field");
+    }
+    *out = f;
+    return WL_OK;
+}
+
+void field_free(wl_field *f) {
+    if (!f) return;
+    if (f->data) (void)hipFree(f->data);
+    delete f;
+}
+
+int ensure_tmp(wl_ctx *c, int prec, wl_field **slot, int parity) {
+    if (*slot) return WL_OK;
+    return field_new(c, parity, prec, 1, slot);
+}
+
+RedArgs red_args(wl_ctx *c) {
+    RedArgs ra;
+    ra.partials = c->partials;
+    ra.ticket = c->ticket;
+    ra.st = c->st;
+    ra.out = c->scratch_d;
+    return ra;
+}
+
+const void *gauge_for(wl_ctx *c, int prec) { return prec == WL_F64 ? c->gauge64 : c->gauge32; }
+
+int ensure_gauge(wl_ctx *c, int prec) {
+    if (!c->gauge_ok) return fail(WL_E_STATE, "no gauge field uploaded (wl_gauge_upload)");
+    if (prec == WL_F64) return WL_OK;
+    if (prec == WL_F32) {
+        if (!c->gauge32) {
+            if (hipMalloc(&c->gauge32, gauge_elems(c->g) * 8) != hipSuccess)
+                return fail(WL_E_OOM, "FP32 gauge copy");
+        }
+        if (!c->gauge32_ok) {
+            HIPCHK(launch_gauge_convert(c->g, WL_F32, c->gauge64, c->gauge32, c->stream));
+            c->gauge32_ok = true;
+        }
+        return WL_OK;
+    }
+    return fail(WL_E_NOTIMPL, "precision %d", prec);
+}
+
+// out_par = a * H_sign in (+ xin), one rhs
+int dslash_one(wl_ctx *c, int prec, void *out, int out_par, const void *in, const void *xin,
+               double a, int sign, int epi, const int *done, int fin) {
+    DslashLaunch L{};
+    L.out = out;
+    L.in = in;
+    L.xin = xin;
+    L.gauge = gauge_for(c, prec);
+    L.a = a;
+    L.out_par = out_par;
+    L.sign = sign;
+    L.epi = epi;
+    L.done = done;
+    L.red = red_args(c);
+    L.fin = fin;
+    HIPCHK(launch_dslash(prec, c->g, L, c->stream));
+    return WL_OK;
+}
+
+// M_s in -> out for even FP64/FP32 fields (one rhs), through the odd temporary tmp:
+// tmp_o = -kappa H_oe in  (Dirac_Wilson_hop_set(..., ODD, sign), wilson.h:275,285)
+// out_e = in + kappa H_eo tmp_o  (Dirac_Wilson_hop(-kappa, out, out, EVEN, sign), :277,287)
+int meo_one(wl_ctx *c, int prec, void *out, const void *in, void *tmp, double kappa, int sign,
+            int epi2, const int *done, int fin2) {
+    int rc = dslash_one(c, prec, tmp, 1, in, nullptr, -kappa, sign, EPI_SET, done, FIN_NONE);
+    if (rc) return rc;
+    return dslash_one(c, prec, out, 0, tmp, in, kappa, sign, epi2, done, fin2);
+}
+
+int check_sign(int sign) { return (sign == 1 || sign == -1) ? WL_OK : fail(WL_E_ARG, "sign must be +-1"); }
+
+}  // namespace
+
+// =================================================================== library ============
+
+extern "C" const char *wl_version(void) { return "hila_wilson 0.1 (gfx950)"; }
+extern "C" const char *wl_last_error(void) { return g_err; }
+
+extern "C" int wl_create(const int dims[4], int antiperiodic_t, int device, wl_ctx **out) {
+    if (!dims || !out) return fail(WL_E_ARG, "null argument");
+    int64_t V = 1;
+    for (int i = 0; i < 4; ++i) {
+        if (dims[i] < 2 || (dims[i] & 1))
+            return fail(WL_E_ARG, "lattice size %d in direction %d: must be even and >= 2", dims[i], i);
+        V *= dims[i];
+    }
+    if (V / 2 > (int64_t)1 << 28) return fail(WL_E_ARG, "lattice too large for 32-bit site indexing");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(WL_E_DEVICE, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(WL_E_ARG, "device %d of %d", device, ndev);
+    wl_ctx *c = new (std::nothrow) wl_ctx;
+    if (!c) return fail(WL_E_OOM, "host allocation");
+    c->device = device;
+    c->V = V;
+    c->antiperiodic = antiperiodic_t ? 1 : 0;
+    for (int i = 0; i < 4; ++i) c->g.L[i] = dims[i];
+    c->g.vh = (int)(V / 2);
+    c->g.stride = (int)((V / 2 + kBlock - 1) / kBlock * kBlock);
+    c->g.antiperiodic = c->antiperiodic;
+    DeviceGuard dg(device);
+    int rc = WL_OK;
+    const int nb = nblocks(c->g);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&c->partials, sizeof(double) * (nb + 64)) != hipSuccess ||
+        hipMalloc(&c->ticket, 256) != hipSuccess ||
+        hipMalloc(&c->st, sizeof(CgState)) != hipSuccess ||
+        hipMalloc(&c->scratch_d, 256) != hipSuccess ||
+        hipHostMalloc(&c->st_host, sizeof(CgState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&c->scratch_h, 256, hipHostMallocDefault) != hipSuccess) {
+        rc = fail(WL_E_DEVICE, "context allocation failed");
+    } else if (hipMemsetAsync(c->ticket, 0, 256, c->stream) != hipSuccess ||
+               hipMemsetAsync(c->st, 0, sizeof(CgState), c->stream) != hipSuccess ||
+               hipStreamSynchronize(c->stream) != hipSuccess) {
+        rc = fail(WL_E_DEVICE, "context initialisation failed");
+    }
+    if (rc) {
+        wl_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return WL_OK;
+}
+
+extern "C" int wl_destroy(wl_ctx *c) {
+    if (!c) return WL_OK;
+    DeviceGuard dg(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (wl_field *f : {c->tmp_odd64, c->tmp_odd32, c->cg_r, c->cg_p, c->cg_Dp, c->cg_DDp})
+        field_free(f);
+    for (void *p : {(void *)c->gauge64, c->gauge32, c->staging, (void *)c->partials,
+                    (void *)c->ticket, (void *)c->st, (void *)c->scratch_d})
+        if (p) (void)hipFree(p);
+    if (c->st_host) (void)hipHostFree(c->st_host);
+    if (c->scratch_h) (void)hipHostFree(c->scratch_h);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return WL_OK;
+}
+
+extern "C" int wl_sync(wl_ctx *c) {
+    if (!c) return fail(WL_E_ARG, "null context");
+    DeviceGuard dg(c->device);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return WL_OK;
+}
+
+extern "C" int64_t wl_volume(const wl_ctx *c) { return c ? c->V : 0; }
+
+extern "C" int wl_gauge_upload(wl_ctx *c, const double *const U[4]) {
+    if (!c || !U) return fail(WL_E_ARG, "null argument");
+    for (int mu = 0; mu < 4; ++mu)
+        if (!U[mu]) return fail(WL_E_ARG, "null gauge direction %d", mu);
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    if (!c->gauge64) {
+        if (hipMalloc(&c->gauge64, gauge_elems(c->g) * 16) != hipSuccess)
+            return fail(WL_E_OOM, "gauge field of %zu bytes", gauge_elems(c->g) * 16);
+        HIPCHK(hipMemsetAsync(c->gauge64, 0, gauge_elems(c->g) * 16, c->stream));
+    }
+    const size_t bytes = (size_t)c->V * 18 * sizeof(double);
+    int rc = ensure_staging(c, bytes);
+    if (rc) return rc;
+    for (int mu = 0; mu < 4; ++mu) {
+        HIPCHK(hipMemcpyAsync(c->staging, U[mu], bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(launch_gauge_to_soa(c->g, mu, (const double *)c->staging, c->gauge64, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->gauge_ok = true;
+    c->gauge32_ok = false;
+    return WL_OK;
+}
+
+// ==================================================================== fields ============
+
+extern "C" int wl_field_create(wl_ctx *c, int parity, int prec, int nrhs, wl_field **out) {
+    if (!c || !out) return fail(WL_E_ARG, "null argument");
+    if (parity != WL_EVEN && parity != WL_ODD) return fail(WL_E_ARG, "parity must be WL_EVEN or WL_ODD");
+    if (prec != WL_F64 && prec != WL_F32) return fail(WL_E_NOTIMPL, "field precision %d", prec);
+    if (nrhs < 1 || nrhs > 64) return fail(WL_E_ARG, "nrhs %d out of range 1..64", nrhs);
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    return field_new(c, parity, prec, nrhs, out);
+}
+
+extern "C" int wl_field_destroy(wl_field *f) {
+    if (!f) return WL_OK;
+    DeviceGuard dg(f->ctx->device);
+    (void)hipStreamSynchronize(f->ctx->stream);
+    field_free(f);
+    return WL_OK;
+}
+
+extern "C" int wl_field_upload(wl_field *f, int rhs, const double *host) {
+    if (!f || !host) return fail(WL_E_ARG, "null argument");
+    if (rhs < 0 || rhs >= f->nrhs) return fail(WL_E_ARG, "rhs %d of %d", rhs, f->nrhs);
+    wl_ctx *c = f->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    const size_t bytes = (size_t)c->g.vh * 24 * sizeof(double);
+    int rc = ensure_staging(c, bytes);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->staging, host, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_aos_to_soa(f->prec, c->g, (const double *)c->staging, f->rhs_ptr(rhs), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return WL_OK;
+}
+
+extern "C" int wl_field_download(const wl_field *f, int rhs, double *host) {
+    if (!f || !host) return fail(WL_E_ARG, "null argument");
+    if (rhs < 0 || rhs >= f->nrhs) return fail(WL_E_ARG, "rhs %d of %d", rhs, f->nrhs);
+    wl_ctx *c = f->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    const size_t bytes = (size_t)c->g.vh * 24 * sizeof(double);
+    int rc = ensure_staging(c, bytes);
+    if (rc) return rc;
+    HIPCHK(launch_soa_to_aos(f->prec, c->g, f->rhs_ptr(rhs), (double *)c->staging, c->stream));
+    HIPCHK(hipMemcpyAsync(host, c->staging, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return WL_OK;
+}
+
+extern "C" int wl_field_zero(wl_field *f) {
+    if (!f) return fail(WL_E_ARG, "null argument");
+    DeviceGuard dg(f->ctx->device);
+    HIPCHK(hipMemsetAsync(f->data, 0, f->bytes, f->ctx->stream));
+    return WL_OK;
+}
+
+extern "C" int wl_field_norm2(const wl_field *f, int rhs, double *out) {
+    if (!f || !out) return fail(WL_E_ARG, "null argument");
+    if (rhs < 0 || rhs >= f->nrhs) return fail(WL_E_ARG, "rhs %d of %d", rhs, f->nrhs);
+    wl_ctx *c = f->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    HIPCHK(launch_norm2(f->prec, c->g, f->rhs_ptr(rhs), red_args(c), FIN_STORE, nullptr, c->stream));
+    HIPCHK(hipMemcpyAsync(c->scratch_h, c->scratch_d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *out = c->scratch_h[0];
+    return WL_OK;
+}
+
+// ================================================================== operator ============
+
+extern "C" int wl_dslash_eo(wl_ctx *c, wl_field *out, const wl_field *in, double kappa, int sign) {
+    if (!c || !out || !in) return fail(WL_E_ARG, "null argument");
+    if (check_sign(sign)) return WL_E_ARG;
+    if (out->ctx != c || in->ctx != c) return fail(WL_E_ARG, "field belongs to another context");
+    if (out->parity == in->parity) return fail(WL_E_ARG, "out and in must have opposite parity");
+    if (out->prec != in->prec || out->nrhs != in->nrhs)
+        return fail(WL_E_ARG, "out and in differ in precision or nrhs");
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    int rc = ensure_gauge(c, in->prec);
+    if (rc) return rc;
+    const int out_par = out->parity == WL_EVEN ? 0 : 1;
+    for (int r = 0; r < in->nrhs; ++r) {
+        rc = dslash_one(c, in->prec, out->rhs_ptr(r), out_par, in->rhs_ptr(r), nullptr, -kappa,
+                        sign, EPI_SET, nullptr, FIN_NONE);
+        if (rc) return rc;
+    }
+    return WL_OK;
+}
+
+extern "C" int wl_meo_apply(wl_ctx *c, wl_field *out, const wl_field *in, double kappa, int sign) {
+    if (!c || !out || !in) return fail(WL_E_ARG, "null argument");
+    if (check_sign(sign)) return WL_E_ARG;
+    if (out->ctx != c || in->ctx != c) return fail(WL_E_ARG, "field belongs to another context");
+    if (out->parity != WL_EVEN || in->parity != WL_EVEN) return fail(WL_E_ARG, "M acts on even fields");
+    if (out == in) return fail(WL_E_ARG, "out and in must be different fields");
+    if (out->prec != in->prec || out->nrhs != in->nrhs)
+        return fail(WL_E_ARG, "out and in differ in precision or nrhs");
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    int rc = ensure_gauge(c, in->prec);
+    if (rc) return rc;
+    wl_field **tmp = in->prec == WL_F64 ? &c->tmp_odd64 : &c->tmp_odd32;
+    rc = ensure_tmp(c, in->prec, tmp, WL_ODD);
+    if (rc) return rc;
+    for (int r = 0; r < in->nrhs; ++r) {
+        rc = meo_one(c, in->prec, out->rhs_ptr(r), in->rhs_ptr(r), (*tmp)->data, kappa, sign,
+                     EPI_AXPY, nullptr, FIN_NONE);
+        if (rc) return rc;
+    }
+    return WL_OK;
+}
+
+// =================================================================== CG =================
+
+namespace {
+
+// Iterations enqueued between two checks of the device `done` flag.
+constexpr int kChunk = 8;
+
+int cg_f64(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int maxiters,
+           wl_cg_stats *stats) {
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    for (wl_field **w : {&c->cg_r, &c->cg_p, &c->cg_Dp, &c->cg_DDp})
+        if ((rc = ensure_tmp(c, WL_F64, w, WL_EVEN))) return rc;
+    if ((rc = ensure_tmp(c, WL_F64, &c->tmp_odd64, WL_ODD))) return rc;
+    void *r = c->cg_r->data, *p = c->cg_p->data, *Dp = c->cg_Dp->data, *DDp = c->cg_DDp->data;
+    void *t = c->tmp_odd64->data;
+    const RedArgs ra = red_args(c);
+    const int *done = &c->st->done;
+
+    CgState init{};
+    init.acc2 = accuracy * accuracy;
+    init.maxiters = maxiters;
+    *c->st_host = init;
+    HIPCHK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    // source_norm and target_rr (conjugate_gradient.h:61-63)
+    HIPCHK(launch_norm2(WL_F64, c->g, b, ra, FIN_SOURCE, nullptr, c->stream));
+    // r = b - M^dag M x ; p = r ; rr = |r|^2  (:65-73)
+    if ((rc = meo_one(c, WL_F64, Dp, x, t, kappa, +1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
+    if ((rc = meo_one(c, WL_F64, DDp, Dp, t, kappa, -1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
+    HIPCHK(launch_cg_init(c->g, b, DDp, r, p, ra, c->stream));
+    HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->st_host->source_norm == 0.0) {
+        // b = 0: the solution is x = 0 (HILA would divide 0 by 0 here)
+        HIPCHK(hipMemsetAsync(x, 0, c->cg_r->bytes, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->converged = 1;
+        }
+        return WL_OK;
+    }
+    if (maxiters <= 0) {
+        c->st_host->done = 2;
+        HIPCHK(hipMemcpyAsync(&c->st->done, &c->st_host->done, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    }
+
+    // main loop (:75-96), kChunk iterations per enqueue; the kernels of an iteration after
+    // convergence return at once.
+    int enqueued = 0;
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    int done_h = c->st_host->done;
+    while (!done_h && enqueued < maxiters) {
+        const int n = std::min(kChunk, maxiters - enqueued);
+        for (int it = 0; it < n; ++it) {
+            // Dp = M p, pDp = |Dp|^2, alpha = rr / pDp
+            if ((rc = meo_one(c, WL_F64, Dp, p, t, kappa, +1, EPI_AXPY_NORM, done, FIN_ALPHA)))
+                return rc;
+            // DDp = M^dag Dp fused with x += alpha p, r -= alpha DDp, rrnew = |r|^2, test, beta
+            if ((rc = dslash_one(c, WL_F64, t, 1, Dp, nullptr, -kappa, -1, EPI_SET, done, FIN_NONE)))
+                return rc;
+            DslashLaunch L{};
+            L.out = nullptr;
+            L.in = t;
+            L.xin = Dp;
+            L.gauge = c->gauge64;
+            L.a = kappa;
+            L.out_par = 0;
+            L.sign = -1;
+            L.epi = EPI_CGUPD;
+            L.done = done;
+            L.x = x;
+            L.p = p;
+            L.r = r;
+            L.red = ra;
+            L.fin = FIN_RRNEW;
+            HIPCHK(launch_dslash(WL_F64, c->g, L, c->stream));
+            // p = beta p + r
+            HIPCHK(launch_cg_xpay(c->g, p, r, c->st, done, c->stream));
+        }
+        enqueued += n;
+        HIPCHK(hipMemcpyAsync(&c->st_host->done, &c->st->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipEventRecord(ev, c->stream));
+        HIPCHK(hipEventSynchronize(ev));
+        done_h = c->st_host->done;
+    }
+    (void)hipEventDestroy(ev);
+
+    // true residual |b - M^dag M x|^2 recomputed in FP64
+    if ((rc = meo_one(c, WL_F64, Dp, x, t, kappa, +1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
+    if ((rc = meo_one(c, WL_F64, DDp, Dp, t, kappa, -1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
+    HIPCHK(launch_diff_norm2(c->g, b, DDp, ra, FIN_TRUE, c->stream));
+    HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const CgState &S = *c->st_host;
+    const bool conv = S.done == 1;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->iterations = S.iter;
+        stats->converged = conv ? 1 : 0;
+        stats->rr_rel = S.rrnew / S.source_norm;
+        stats->true_resid = std::sqrt(S.true_rr / S.source_norm);
+        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (!conv) return fail(WL_E_NOCONV, "CG did not converge in %d iterations (rr_rel %.3e)", S.iter,
+                           S.rrnew / S.source_norm);
+    return WL_OK;
+}
+
+}  // namespace
+
+extern "C" int wl_cg_solve(wl_ctx *c, wl_field *x, const wl_field *b, double kappa, double accuracy,
+                           int maxiters, wl_cg_stats *stats) {
+    if (!c || !x || !b) return fail(WL_E_ARG, "null argument");
+    if (x->ctx != c || b->ctx != c) return fail(WL_E_ARG, "field belongs to another context");
+    if (x->parity != WL_EVEN || b->parity != WL_EVEN) return fail(WL_E_ARG, "CG acts on even fields");
+    if (x == b) return fail(WL_E_ARG, "x and b must be different fields");
+    if (x->prec != WL_F64 || b->prec != WL_F64) return fail(WL_E_ARG, "wl_cg_solve takes FP64 fields");
+    if (x->nrhs != b->nrhs) return fail(WL_E_ARG, "x and b differ in nrhs");
+    if (!(accuracy > 0)) return fail(WL_E_ARG, "accuracy must be > 0");
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    int rc = ensure_gauge(c, WL_F64);
+    if (rc) return rc;
+    int worst = WL_OK;
+    for (int r = 0; r < b->nrhs; ++r) {
+        wl_cg_stats s{};
+        rc = cg_f64(c, x->rhs_ptr(r), b->rhs_ptr(r), kappa, accuracy, maxiters, &s);
+        if (rc && rc != WL_E_NOCONV) return rc;
+        if (rc) worst = rc;
+        if (stats) stats[r] = s;
+    }
+    return worst;
+}
+
+// ================================================================ diagnostics ===========
+
+extern "C" int wl_time_op(wl_ctx *c, int what, wl_field *out, const wl_field *in, double kappa,
+                          int sign, int warmup, int iters, double *ms_per_launch) {
+    if (!c || !out || !in || !ms_per_launch || iters < 1 || warmup < 0)
+        return fail(WL_E_ARG, "bad argument");
+    auto once = [&]() -> int {
+        return what == 0 ? wl_dslash_eo(c, out, in, kappa, sign) : wl_meo_apply(c, out, in, kappa, sign);
+    };
+    if (what != 0 && what != 1) return fail(WL_E_ARG, "what must be 0 (hopping term) or 1 (M)");
+    for (int i = 0; i < warmup; ++i)
+        if (int rc = once()) return rc;
+    DeviceGuard dg(c->device);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < iters; ++i)
+        if (int rc = once()) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc;
+        }
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_launch = (double)ms / iters;
+    return WL_OK;
+}

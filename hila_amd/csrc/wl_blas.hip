@@ -1,0 +1,234 @@
+// Layout conversion, BLAS-1 and reduction kernels of the Wilson CG (gfx950).
+//
+// The reductions replace HILA's per-workgroup shared-memory tree followed by host-side
+// summation of the partials (hilapp codegen_gpu.cpp:1033-1088, backend_gpu/
+// gpu_templated_ops.h:130-162): partials are summed on the device by the last workgroup, in
+// workgroup order, and the result goes straight into the device-resident CG state.
+
+#include "wl_internal.h"
+
+namespace wl {
+
+// ------------------------------------------------------------------- layout ------------
+
+// HILA AoS (V/2 sites x 24 doubles) -> component planes
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_aos_to_soa(const double *__restrict__ aos,
+                                                       cx<R> *__restrict__ out, int vh, int S) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= vh) return;
+    const double *src = aos + (size_t)24 * k;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) out[c * S + k] = {(R)src[2 * c], (R)src[2 * c + 1]};
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_soa_to_aos(const cx<R> *__restrict__ in,
+                                                       double *__restrict__ aos, int vh, int S) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= vh) return;
+    double *dst = aos + (size_t)24 * k;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+        const cx<R> v = in[c * S + k];
+        dst[2 * c] = (double)v.x;
+        dst[2 * c + 1] = (double)v.y;
+    }
+}
+
+// One direction of HILA's Field<SU<3,double>> (V sites x 18 doubles, even sites first) ->
+// gauge planes of both parities.
+__global__ __launch_bounds__(kBlock) void k_gauge_to_soa(const double *__restrict__ aos,
+                                                         cx<double> *__restrict__ gauge, int mu,
+                                                         int vh, int S) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= 2 * vh) return;
+    const int p = s >= vh;
+    const int k = s - p * vh;
+    const double *src = aos + (size_t)18 * s;
+    cx<double> *dst = gauge + (size_t)((p * 4 + mu) * 9) * S;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) dst[e * S + k] = {src[2 * e], src[2 * e + 1]};
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_convert(const cx<double> *__restrict__ in,
+                                                    cx<R> *__restrict__ out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const cx<double> v = in[i];
+    out[i] = {(R)v.x, (R)v.y};
+}
+
+// ------------------------------------------------------------------ reductions ---------
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_norm2(const cx<R> *__restrict__ f, int vh, int S,
+                                                  RedArgs ra, int fin, const int *done) {
+    if (done && *done) return;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    double s = 0;
+    if (k < vh) {
+#pragma unroll
+        for (int c = 0; c < 12; ++c) s += (double)norm2(f[c * S + k]);
+    }
+    reduce_finish(s, fin, ra);
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_init(const cx<double> *__restrict__ b,
+                                                    const cx<double> *__restrict__ ddp,
+                                                    cx<double> *__restrict__ r,
+                                                    cx<double> *__restrict__ p, int vh, int S,
+                                                    RedArgs ra) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    double s = 0;
+    if (k < vh) {
+#pragma unroll
+        for (int c = 0; c < 12; ++c) {
+            const int e = c * S + k;
+            const cx<double> v = b[e] - ddp[e];
+            r[e] = v;
+            p[e] = v;
+            s += norm2(v);
+        }
+    }
+    reduce_finish(s, FIN_RR, ra);
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_xpay(cx<double> *__restrict__ p,
+                                                    const cx<double> *__restrict__ r,
+                                                    const CgState *st, int vh, int S) {
+    if (st->done) return;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= vh) return;
+    const double beta = st->beta;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+        const int e = c * S + k;
+        p[e] = beta * p[e] + r[e];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_diff_norm2(const cx<double> *__restrict__ b,
+                                                       const cx<double> *__restrict__ v, int vh,
+                                                       int S, RedArgs ra, int fin) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    double s = 0;
+    if (k < vh) {
+#pragma unroll
+        for (int c = 0; c < 12; ++c) {
+            const int e = c * S + k;
+            s += norm2(b[e] - v[e]);
+        }
+    }
+    reduce_finish(s, fin, ra);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_copy(const cx<R> *__restrict__ src,
+                                                 cx<R> *__restrict__ dst, int vh, int S) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= vh) return;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) dst[c * S + k] = src[c * S + k];
+}
+
+// ------------------------------------------------------------------- launchers ---------
+
+static inline dim3 grid_sites(const Geom &g) { return dim3(nblocks(g)); }
+
+hipError_t launch_aos_to_soa(int prec, const Geom &g, const double *aos, void *planes,
+                             hipStream_t s) {
+    if (prec == WL_F64)
+        hipLaunchKernelGGL(k_aos_to_soa<double>, grid_sites(g), dim3(kBlock), 0, s, aos,
+                           (cx<double> *)planes, g.vh, g.stride);
+    else if (prec == WL_F32)
+        hipLaunchKernelGGL(k_aos_to_soa<float>, grid_sites(g), dim3(kBlock), 0, s, aos,
+                           (cx<float> *)planes, g.vh, g.stride);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(int prec, const Geom &g, const void *planes, double *aos,
+                             hipStream_t s) {
+    if (prec == WL_F64)
+        hipLaunchKernelGGL(k_soa_to_aos<double>, grid_sites(g), dim3(kBlock), 0, s,
+                           (const cx<double> *)planes, aos, g.vh, g.stride);
+    else if (prec == WL_F32)
+        hipLaunchKernelGGL(k_soa_to_aos<float>, grid_sites(g), dim3(kBlock), 0, s,
+                           (const cx<float> *)planes, aos, g.vh, g.stride);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_gauge_to_soa(const Geom &g, int mu, const double *aos_full, void *gauge64,
+                               hipStream_t s) {
+    const int n = 2 * g.vh;
+    hipLaunchKernelGGL(k_gauge_to_soa, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       aos_full, (cx<double> *)gauge64, mu, g.vh, g.stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_gauge_convert(const Geom &g, int prec, const void *gauge64, void *gauge_lo,
+                                hipStream_t s) {
+    const size_t n = gauge_elems(g);
+    const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+    if (prec == WL_F32)
+        hipLaunchKernelGGL(k_convert<float>, grid, dim3(kBlock), 0, s,
+                           (const cx<double> *)gauge64, (cx<float> *)gauge_lo, n);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_norm2(int prec, const Geom &g, const void *f, const RedArgs &ra, int fin,
+                        const int *done, hipStream_t s) {
+    if (prec == WL_F64)
+        hipLaunchKernelGGL(k_norm2<double>, grid_sites(g), dim3(kBlock), 0, s,
+                           (const cx<double> *)f, g.vh, g.stride, ra, fin, done);
+    else if (prec == WL_F32)
+        hipLaunchKernelGGL(k_norm2<float>, grid_sites(g), dim3(kBlock), 0, s,
+                           (const cx<float> *)f, g.vh, g.stride, ra, fin, done);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_init(const Geom &g, const void *b, const void *DDp, void *r, void *p,
+                          const RedArgs &ra, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_init, grid_sites(g), dim3(kBlock), 0, s, (const cx<double> *)b,
+                       (const cx<double> *)DDp, (cx<double> *)r, (cx<double> *)p, g.vh,
+                       g.stride, ra);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_xpay(const Geom &g, void *p, const void *r, const CgState *st,
+                          const int *done, hipStream_t s) {
+    (void)done;
+    hipLaunchKernelGGL(k_cg_xpay, grid_sites(g), dim3(kBlock), 0, s, (cx<double> *)p,
+                       (const cx<double> *)r, st, g.vh, g.stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_diff_norm2(const Geom &g, const void *b, const void *v, const RedArgs &ra,
+                             int fin, hipStream_t s) {
+    hipLaunchKernelGGL(k_diff_norm2, grid_sites(g), dim3(kBlock), 0, s, (const cx<double> *)b,
+                       (const cx<double> *)v, g.vh, g.stride, ra, fin);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(int prec, const Geom &g, const void *src, void *dst, hipStream_t s) {
+    if (prec == WL_F64)
+        hipLaunchKernelGGL(k_copy<double>, grid_sites(g), dim3(kBlock), 0, s,
+                           (const cx<double> *)src, (cx<double> *)dst, g.vh, g.stride);
+    else if (prec == WL_F32)
+        hipLaunchKernelGGL(k_copy<float>, grid_sites(g), dim3(kBlock), 0, s,
+                           (const cx<float> *)src, (cx<float> *)dst, g.vh, g.stride);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace wl

@@ -1,0 +1,313 @@
+// Device-side building blocks of the gfx950 Wilson kernels: complex arithmetic, the spin
+// projection / reconstruction of the Wilson hop, SU(3) x half-spinor products, checkerboard
+// geometry and wave64 reductions.
+//
+// Spin conventions follow libraries/datatypes/wilson_vector.h:439-567 (HalfWilsonVector
+// projection and expand, unnormalised, gamma map {g1,g2,g3,g0}); SU(3) algebra follows
+// matrix.h:1118-1137 (adjoint) and :2599-2622 (matrix x vector).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wl {
+
+constexpr int kBlock = 256;  // threads per workgroup = 4 wave64
+
+// ------------------------------------------------------------------ complex numbers ----
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cx {
+    R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) {
+    return {a.x + b.x, a.y + b.y};
+}
+template <typename R>
+__device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) {
+    return {a.x - b.x, a.y - b.y};
+}
+template <typename R>
+__device__ __forceinline__ cx<R> operator*(R s, cx<R> a) {
+    return {s * a.x, s * a.y};
+}
+// a * b
+template <typename R>
+__device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// conj(a) * b
+template <typename R>
+__device__ __forceinline__ cx<R> cmulc(cx<R> a, cx<R> b) {
+    return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+// acc + a * b
+template <typename R>
+__device__ __forceinline__ cx<R> cfma(cx<R> acc, cx<R> a, cx<R> b) {
+    acc.x += a.x * b.x - a.y * b.y;
+    acc.y += a.x * b.y + a.y * b.x;
+    return acc;
+}
+// acc + conj(a) * b
+template <typename R>
+__device__ __forceinline__ cx<R> cfmac(cx<R> acc, cx<R> a, cx<R> b) {
+    acc.x += a.x * b.x + a.y * b.y;
+    acc.y += a.x * b.y - a.y * b.x;
+    return acc;
+}
+// s * i * a  with s = +-1 folded at compile time
+template <int S, typename R>
+__device__ __forceinline__ cx<R> si(cx<R> a) {
+    return S > 0 ? cx<R>{-a.y, a.x} : cx<R>{a.y, -a.x};
+}
+template <int S, typename R>
+__device__ __forceinline__ cx<R> sr(cx<R> a) {
+    return S > 0 ? a : cx<R>{-a.x, -a.y};
+}
+template <typename R>
+__device__ __forceinline__ R norm2(cx<R> a) {
+    return a.x * a.x + a.y * a.y;
+}
+
+template <typename R>
+using Spinor = cx<R>[4][3];
+template <typename R>
+using Half = cx<R>[2][3];
+template <typename R>
+using Link = cx<R>[3][3];
+
+// ------------------------------------------------------- spin projection (1 + S g_mu) ----
+// HalfWilsonVector(w, dir, S): the two upper spin rows of (1 + S g_dir) w.
+template <int MU, int S, typename R>
+__device__ __forceinline__ void project(const cx<R> (&w)[4][3], cx<R> (&h)[2][3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if constexpr (MU == 0) {  // g1
+            h[0][c] = w[0][c] + si<S>(w[3][c]);
+            h[1][c] = w[1][c] + si<S>(w[2][c]);
+        } else if constexpr (MU == 1) {  // g2
+            h[0][c] = w[0][c] - sr<S>(w[3][c]);
+            h[1][c] = w[1][c] + sr<S>(w[2][c]);
+        } else if constexpr (MU == 2) {  // g3
+            h[0][c] = w[0][c] + si<S>(w[2][c]);
+            h[1][c] = w[1][c] - si<S>(w[3][c]);
+        } else {  // g0
+            h[0][c] = w[0][c] + sr<S>(w[2][c]);
+            h[1][c] = w[1][c] + sr<S>(w[3][c]);
+        }
+    }
+}
+
+// acc += expand(h, MU, S): rebuild the lower spin rows of (1 + S g_mu) from the upper two.
+template <int MU, int S, typename R>
+__device__ __forceinline__ void expand_add(cx<R> (&acc)[4][3], const cx<R> (&h)[2][3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        acc[0][c] = acc[0][c] + h[0][c];
+        acc[1][c] = acc[1][c] + h[1][c];
+        if constexpr (MU == 0) {
+            acc[2][c] = acc[2][c] - si<S>(h[1][c]);
+            acc[3][c] = acc[3][c] - si<S>(h[0][c]);
+        } else if constexpr (MU == 1) {
+            acc[2][c] = acc[2][c] + sr<S>(h[1][c]);
+            acc[3][c] = acc[3][c] - sr<S>(h[0][c]);
+        } else if constexpr (MU == 2) {
+            acc[2][c] = acc[2][c] - si<S>(h[0][c]);
+            acc[3][c] = acc[3][c] + si<S>(h[1][c]);
+        } else {
+            acc[2][c] = acc[2][c] + sr<S>(h[0][c]);
+            acc[3][c] = acc[3][c] + sr<S>(h[1][c]);
+        }
+    }
+}
+
+// v = U h   (each of the two colour vectors)
+template <typename R>
+__device__ __forceinline__ void su3_mul(const cx<R> (&u)[3][3], const cx<R> (&h)[2][3],
+                                        cx<R> (&v)[2][3]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            cx<R> a = cmul(u[i][0], h[s][0]);
+            a = cfma(a, u[i][1], h[s][1]);
+            v[s][i] = cfma(a, u[i][2], h[s][2]);
+        }
+}
+
+// v = U^dag h
+template <typename R>
+__device__ __forceinline__ void su3_adj_mul(const cx<R> (&u)[3][3], const cx<R> (&h)[2][3],
+                                            cx<R> (&v)[2][3]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            cx<R> a = cmulc(u[0][i], h[s][0]);
+            a = cfmac(a, u[1][i], h[s][1]);
+            v[s][i] = cfmac(a, u[2][i], h[s][2]);
+        }
+}
+
+// --------------------------------------------------------------------- geometry -------
+// Checkerboard site k of parity p (0 even, 1 odd) holds the site with lexicographic index
+// 2k or 2k+1, whichever has parity p (HILA's EVEN_SITES_FIRST index, lattice.cpp:171-186).
+struct Geom {
+    int L[4];        // Lx, Ly, Lz, Lt (all even)
+    int vh;          // sites per parity
+    int stride;      // elements between two component planes of a field (>= vh, padded)
+    int antiperiodic;
+};
+
+struct SiteNb {
+    int fwd[4], bwd[4];   // checkerboard index of x+mu / x-mu (opposite parity)
+    bool fwd_edge_t, bwd_edge_t;
+};
+
+__device__ __forceinline__ SiteNb neighbours(const Geom &g, int k, int p) {
+    const unsigned Lx = g.L[0], Ly = g.L[1], Lz = g.L[2], Lt = g.L[3];
+    const unsigned q = 2u * (unsigned)k;
+    const unsigned x0 = q % Lx;
+    unsigned r = q / Lx;
+    const unsigned y = r % Ly;
+    r /= Ly;
+    const unsigned z = r % Lz;
+    const unsigned t = r / Lz;
+    const unsigned odd = ((unsigned)p + y + z + t) & 1u;
+    const unsigned x = x0 + odd;
+    const unsigned idx = q + odd;
+    const unsigned sy = Lx, sz = Lx * Ly, st = Lx * Ly * Lz;
+    SiteNb n;
+    n.fwd[0] = (x == Lx - 1 ? idx - (Lx - 1) : idx + 1) >> 1;
+    n.bwd[0] = (x == 0 ? idx + (Lx - 1) : idx - 1) >> 1;
+    n.fwd[1] = (y == Ly - 1 ? idx - sy * (Ly - 1) : idx + sy) >> 1;
+    n.bwd[1] = (y == 0 ? idx + sy * (Ly - 1) : idx - sy) >> 1;
+    n.fwd[2] = (z == Lz - 1 ? idx - sz * (Lz - 1) : idx + sz) >> 1;
+    n.bwd[2] = (z == 0 ? idx + sz * (Lz - 1) : idx - sz) >> 1;
+    n.fwd[3] = (t == Lt - 1 ? idx - st * (Lt - 1) : idx + st) >> 1;
+    n.bwd[3] = (t == 0 ? idx + st * (Lt - 1) : idx - st) >> 1;
+    n.fwd_edge_t = (t == Lt - 1);
+    n.bwd_edge_t = (t == 0);
+    return n;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b -> XCD b % 8).  Map them so
+// that each XCD walks one contiguous eighth of the checkerboard: neighbouring sites (and the
+// spinors they share) then stay in one XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    if (nb & 7) return b;
+    return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// ------------------------------------------------------------------- reductions ---------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over the workgroup in a fixed order; the result is valid in thread 0.  Contains
+// barriers: every thread of the workgroup must call it.
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double part[kBlock / 64];
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) part[w] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < kBlock / 64; ++i) s += part[i];
+    }
+    __syncthreads();
+    return s;
+}
+
+__device__ __forceinline__ double load_coherent(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Device-resident scalars of one CG solve (conjugate_gradient.h:55-57), updated by the last
+// workgroup of each reduction kernel so that no scalar leaves the device inside the loop.
+struct CgState {
+    double source_norm, target_rr, rr, pDp, rrnew, alpha, beta, acc2;
+    double true_rr, scratch;
+    int iter, done, maxiters, nreliable;
+};
+
+// What the last workgroup of a reduction does with the reduced sum.
+enum Fin : int {
+    FIN_NONE = 0,
+    FIN_STORE = 1,     // *out = s
+    FIN_SOURCE = 2,    // source_norm = s; target_rr = acc2 * s
+    FIN_RR = 3,        // rr = s
+    FIN_ALPHA = 4,     // pDp = s; alpha = rr / pDp
+    FIN_RRNEW = 5,     // rrnew = s; convergence test and beta (conjugate_gradient.h:87-95)
+    FIN_TRUE = 6,      // true_rr = s
+};
+
+struct RedArgs {
+    double *partials;   // one slot per workgroup
+    unsigned *ticket;   // zero between reductions
+    CgState *st;
+    double *out;        // FIN_STORE target
+};
+
+__device__ __forceinline__ void fin_apply(int fin, double s, const RedArgs &ra) {
+    CgState *st = ra.st;
+    switch (fin) {
+    case FIN_STORE: *ra.out = s; break;
+    case FIN_SOURCE:
+        st->source_norm = s;
+        st->target_rr = st->acc2 * s;
+        break;
+    case FIN_RR: st->rr = s; break;
+    case FIN_ALPHA:
+        st->pDp = s;
+        st->alpha = st->rr / s;
+        break;
+    case FIN_RRNEW:
+        st->rrnew = s;
+        st->iter += 1;
+        if (s < st->target_rr) {
+            st->done = 1;
+        } else {
+            st->beta = s / st->rr;
+            st->rr = s;
+            if (st->iter >= st->maxiters) st->done = 2;
+        }
+        break;
+    case FIN_TRUE: st->true_rr = s; break;
+    default: break;
+    }
+}
+
+// Publish this workgroup's partial sum; the last workgroup to arrive sums all partials in
+// workgroup order (deterministic) and applies `fin`.  Every thread must call it.
+__device__ __forceinline__ void reduce_finish(double v, int fin, const RedArgs &ra) {
+    __shared__ int am_last;
+    const double bs = block_sum(v);
+    const int nblk = gridDim.x;
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(ra.partials + blockIdx.x, bs, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned t = atomicAdd(ra.ticket, 1u);
+        am_last = (t == (unsigned)nblk - 1u);
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    double s = 0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += load_coherent(ra.partials + i);
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+        fin_apply(fin, s, ra);
+        __hip_atomic_store(ra.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+}  // namespace wl

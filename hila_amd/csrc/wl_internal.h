@@ -1,0 +1,119 @@
+// Host-side internals shared by the translation units of libhila_wilson.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#include "wl_device.h"
+#include "wilson_mi355x.h"
+
+namespace wl {
+
+// Device layout of a Wilson-vector field of one parity: for every rhs, 12 component planes
+// (spin-major, colour-minor) of `stride` complex numbers; site k of the parity is element k of
+// each plane.  Gauge field: for parity p, direction mu, matrix entry e = 3*row + col, a plane of
+// `stride` complex numbers: gauge[((p*4 + mu)*9 + e)*stride + k] = U_mu(site k of parity p).
+inline size_t field_elems(const Geom &g, int nrhs) { return (size_t)nrhs * 12 * g.stride; }
+inline size_t gauge_elems(const Geom &g) { return (size_t)2 * 4 * 9 * g.stride; }
+
+inline int prec_bytes(int prec) { return prec == WL_F64 ? 16 : (prec == WL_F32 ? 8 : 4); }
+
+// Which epilogue a Dslash launch applies to a = coefficient times the hopping sum H in.
+enum Epi : int {
+    EPI_SET = 0,        // out = a H in                           (Dirac_Wilson_hop_set)
+    EPI_AXPY = 1,       // out = xin + a H in                     (second half of M)
+    EPI_AXPY_NORM = 2,  // out = xin + a H in, reduce ||out||^2   (M p with pDp fused)
+    EPI_CGUPD = 3,      // DDp = xin + a H in; x += alpha p; r -= alpha DDp; reduce ||r||^2
+};
+
+struct DslashLaunch {
+    void *out;
+    const void *in;
+    const void *xin;
+    const void *gauge;
+    double a;
+    int out_par;   // 0 even, 1 odd
+    int sign;      // +1 apply, -1 dagger
+    int epi;
+    const int *done;  // skip the launch body when *done != 0 (CG loop); may be null
+    // EPI_CGUPD operands (FP64 even fields)
+    void *x;
+    const void *p;
+    void *r;
+    RedArgs red;
+    int fin;
+};
+
+// Kernel launchers (wl_dslash.hip / wl_blas.hip).  All enqueue on `s` and return hipError_t.
+hipError_t launch_dslash(int prec, const Geom &g, const DslashLaunch &L, hipStream_t s);
+
+hipError_t launch_aos_to_soa(int prec, const Geom &g, const double *aos, void *planes,
+                             hipStream_t s);
+hipError_t launch_soa_to_aos(int prec, const Geom &g, const void *planes, double *aos,
+                             hipStream_t s);
+hipError_t launch_gauge_to_soa(const Geom &g, int mu, const double *aos_full, void *gauge64,
+                               hipStream_t s);
+hipError_t launch_gauge_convert(const Geom &g, int prec, const void *gauge64, void *gauge_lo,
+                                hipStream_t s);
+// sum over sites of ||f||^2 (FP64 accumulate), finished with `fin`
+hipError_t launch_norm2(int prec, const Geom &g, const void *f, const RedArgs &ra, int fin,
+                        const int *done, hipStream_t s);
+// r = b - DDp; p = r; reduce ||r||^2 -> FIN_RR   (conjugate_gradient.h:67-72)
+hipError_t launch_cg_init(const Geom &g, const void *b, const void *DDp, void *r, void *p,
+                          const RedArgs &ra, hipStream_t s);
+// p = beta p + r, beta from the CG state (conjugate_gradient.h:93-94)
+hipError_t launch_cg_xpay(const Geom &g, void *p, const void *r, const CgState *st,
+                          const int *done, hipStream_t s);
+// reduce ||b - v||^2 with `fin`
+hipError_t launch_diff_norm2(const Geom &g, const void *b, const void *v, const RedArgs &ra,
+                             int fin, hipStream_t s);
+// copy f64 even field v into the all-zero-if-null convention: y = v
+hipError_t launch_copy(int prec, const Geom &g, const void *src, void *dst, hipStream_t s);
+
+int nblocks(const Geom &g);
+
+}  // namespace wl
+
+// ---------------------------------------------------------------- ABI objects ---------
+
+struct wl_field {
+    wl_ctx *ctx;
+    int parity;  // WL_EVEN / WL_ODD
+    int prec;
+    int nrhs;
+    void *data;
+    size_t rhs_elems;  // complex elements per rhs = 12 * stride
+    size_t bytes;
+    void *rhs_ptr(int r) const {
+        return (char *)data + (size_t)r * rhs_elems * wl::prec_bytes(prec);
+    }
+};
+
+struct wl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    wl::Geom g{};
+    int64_t V = 0;
+    int antiperiodic = 1;
+    void *gauge64 = nullptr;
+    void *gauge32 = nullptr;
+    bool gauge_ok = false;
+    bool gauge32_ok = false;
+    void *staging = nullptr;
+    size_t staging_bytes = 0;
+    double *partials = nullptr;
+    unsigned *ticket = nullptr;
+    wl::CgState *st = nullptr;
+    wl::CgState *st_host = nullptr;  // pinned
+    double *scratch_d = nullptr;     // FIN_STORE target
+    double *scratch_h = nullptr;     // pinned
+    // work fields (created on first use)
+    wl_field *tmp_odd64 = nullptr;   // odd temporary of M
+    wl_field *tmp_odd32 = nullptr;
+    wl_field *cg_r = nullptr, *cg_p = nullptr, *cg_Dp = nullptr, *cg_DDp = nullptr;
+    std::mutex mu;
+};

@@ -1,0 +1,301 @@
+"""Python mirror of HILA's Wilson solve interface over the C ABI of libhila_wilson.so.
+
+The class and method names follow the reference call surface:
+
+    Dirac_Wilson_evenodd(kappa, gauge)          libraries/dirac/wilson.h:234-269
+        .apply(in, out) / .dagger(in, out)      wilson.h:272-289
+        .par == EVEN, .kappa
+    CG(op, accuracy=1e-12, maxiters=10000)      libraries/dirac/conjugate_gradient.h:19-40
+        .apply(in, out)   out = initial guess & result                     :46-104
+
+Fields here are device-resident (`Field`), created on a `Lattice` context; host data is
+exchanged in HILA's AoS layout (sites x 24 doubles, EVEN_SITES_FIRST order).  Every call goes
+to the HIP library; there is no CPU fallback: without the built library, or without a GPU,
+the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libhila_wilson.so")
+
+# status codes and enums of include/wilson_mi355x.h
+WL_OK, WL_E_ARG, WL_E_OOM, WL_E_DEVICE, WL_E_NOCONV, WL_E_STATE, WL_E_NOTIMPL = 0, -1, -2, -3, -4, -5, -6
+EVEN, ODD = 1, 2
+F64, F32, F16 = 0, 1, 2
+
+EXPORTED_SYMBOLS = [
+    "wl_version", "wl_last_error", "wl_create", "wl_destroy", "wl_sync", "wl_volume",
+    "wl_gauge_upload", "wl_field_create", "wl_field_destroy", "wl_field_upload",
+    "wl_field_download", "wl_field_zero", "wl_field_norm2", "wl_dslash_eo", "wl_meo_apply",
+    "wl_cg_solve", "wl_time_op", "wl_gen_gauge", "wl_gen_spinor", "wl_gen_point",
+]
+
+
+class WilsonError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class CGStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("rr_rel", C.c_double),
+                ("true_resid", C.c_double), ("ms", C.c_double), ("reliable_updates", C.c_int),
+                ("pad_", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
+
+
+_lib = None
+
+
+def lib():
+    """Load libhila_wilson.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m hila_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, i, d, i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+        ip4 = C.POINTER(C.c_int)
+        L.wl_version.restype = C.c_char_p
+        L.wl_last_error.restype = C.c_char_p
+        L.wl_create.argtypes = [ip4, i, i, C.POINTER(vp)]
+        L.wl_destroy.argtypes = [vp]
+        L.wl_sync.argtypes = [vp]
+        L.wl_volume.argtypes = [vp]
+        L.wl_volume.restype = i64
+        L.wl_gauge_upload.argtypes = [vp, C.POINTER(vp)]
+        L.wl_field_create.argtypes = [vp, i, i, i, C.POINTER(vp)]
+        L.wl_field_destroy.argtypes = [vp]
+        L.wl_field_upload.argtypes = [vp, i, vp]
+        L.wl_field_download.argtypes = [vp, i, vp]
+        L.wl_field_zero.argtypes = [vp]
+        L.wl_field_norm2.argtypes = [vp, i, C.POINTER(d)]
+        L.wl_dslash_eo.argtypes = [vp, vp, vp, d, i]
+        L.wl_meo_apply.argtypes = [vp, vp, vp, d, i]
+        L.wl_cg_solve.argtypes = [vp, vp, vp, d, d, i, C.POINTER(CGStats)]
+        L.wl_time_op.argtypes = [vp, i, vp, vp, d, i, i, i, C.POINTER(d)]
+        L.wl_gen_gauge.argtypes = [ip4, i, d, C.c_uint64, C.POINTER(vp)]
+        L.wl_gen_spinor.argtypes = [ip4, C.c_uint64, i, vp]
+        L.wl_gen_point.argtypes = [ip4, C.c_uint64, i, i, vp, C.POINTER(i64)]
+        for name in EXPORTED_SYMBOLS:
+            if name not in ("wl_version", "wl_last_error", "wl_volume"):
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc, allow=()):
+    if rc != WL_OK and rc not in allow:
+        raise WilsonError(rc, lib().wl_last_error().decode())
+    return rc
+
+
+def _dims(dims):
+    d = tuple(int(x) for x in dims)
+    if len(d) != 4:
+        raise ValueError("dims must have 4 entries (Lx, Ly, Lz, Lt)")
+    return (C.c_int * 4)(*d)
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+# ------------------------------------------------------------------ seeded inputs ------
+
+def gen_gauge(dims, kind="smooth", eps=0.3, seed=1):
+    """Gauge field in HILA's layout, (4, V, 18) float64.  kind: 'unit', 'haar' or 'smooth'
+    (exp(i eps H), H traceless Hermitian with unit-variance Gaussian entries)."""
+    kinds = {"unit": 0, "haar": 1, "smooth": 2}
+    V = int(np.prod(dims))
+    U = np.empty((4, V, 18), dtype=np.float64)
+    ptrs = (C.c_void_p * 4)(*[U[m].ctypes.data for m in range(4)])
+    check(lib().wl_gen_gauge(_dims(dims), kinds[kind], float(eps), int(seed), ptrs))
+    return U
+
+
+def gen_spinor(dims, seed, parity_mask=EVEN):
+    """Gaussian Wilson-vector field (V, 24), zero outside parity_mask (1 even, 2 odd, 3 all)."""
+    V = int(np.prod(dims))
+    f = np.empty((V, 24), dtype=np.float64)
+    check(lib().wl_gen_spinor(_dims(dims), int(seed), int(parity_mask), _ptr(f)))
+    return f
+
+
+def gen_point(dims, seed, spin, colour):
+    """Point source (V, 24) at a seeded site; returns (field, EVEN_SITES_FIRST site index)."""
+    V = int(np.prod(dims))
+    f = np.empty((V, 24), dtype=np.float64)
+    site = C.c_int64(0)
+    check(lib().wl_gen_point(_dims(dims), int(seed), int(spin), int(colour), _ptr(f),
+                             C.byref(site)))
+    return f, int(site.value)
+
+
+# ------------------------------------------------------------------------ objects ------
+
+class Lattice:
+    """A lattice geometry on one GPU (wl_ctx): HILA's lattice.setup() for this path."""
+
+    def __init__(self, dims, antiperiodic_t=True, device=0):
+        self.dims = tuple(int(x) for x in dims)
+        self.V = int(np.prod(self.dims))
+        self.Vh = self.V // 2
+        h = C.c_void_p()
+        check(lib().wl_create(_dims(self.dims), 1 if antiperiodic_t else 0, int(device),
+                              C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise WilsonError(WL_E_STATE, "lattice context closed")
+        return self._h
+
+    def upload_gauge(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != (4, self.V, 18):
+            raise ValueError(f"gauge must have shape (4, {self.V}, 18)")
+        ptrs = (C.c_void_p * 4)(*[U[m].ctypes.data for m in range(4)])
+        check(lib().wl_gauge_upload(self.handle, ptrs))
+
+    def sync(self):
+        check(lib().wl_sync(self.handle))
+
+    def close(self):
+        if self._h is not None:
+            lib().wl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Field:
+    """A device Wilson-vector field on one parity (nrhs vectors, precision prec)."""
+
+    def __init__(self, lat: Lattice, parity=EVEN, prec=F64, nrhs=1):
+        self.lat = lat
+        self.parity = parity
+        self.prec = prec
+        self.nrhs = nrhs
+        h = C.c_void_p()
+        check(lib().wl_field_create(lat.handle, parity, prec, nrhs, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _block(self, host):
+        host = np.asarray(host, dtype=np.float64)
+        Vh = self.lat.Vh
+        if host.shape == (self.lat.V, 24):  # full HILA field: take this parity's block
+            host = host[:Vh] if self.parity == EVEN else host[Vh:]
+        if host.shape != (Vh, 24):
+            raise ValueError(f"host field must be (V, 24) or (V/2, 24); got {host.shape}")
+        return np.ascontiguousarray(host)
+
+    def upload(self, host, rhs=0):
+        b = self._block(host)
+        check(lib().wl_field_upload(self._h, rhs, _ptr(b)))
+        return self
+
+    def download(self, rhs=0):
+        out = np.empty((self.lat.Vh, 24), dtype=np.float64)
+        check(lib().wl_field_download(self._h, rhs, _ptr(out)))
+        return out
+
+    def to_full(self, rhs=0):
+        """HILA full-lattice AoS array (V, 24), other parity zero."""
+        full = np.zeros((self.lat.V, 24))
+        blk = self.download(rhs)
+        if self.parity == EVEN:
+            full[: self.lat.Vh] = blk
+        else:
+            full[self.lat.Vh:] = blk
+        return full
+
+    def zero(self):
+        check(lib().wl_field_zero(self._h))
+        return self
+
+    def norm2(self, rhs=0):
+        d = C.c_double(0)
+        check(lib().wl_field_norm2(self._h, rhs, C.byref(d)))
+        return d.value
+
+    def close(self):
+        if self._h is not None:
+            lib().wl_field_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Dirac_Wilson_evenodd:
+    """HILA's Dirac_Wilson_evenodd<SU<3,double>> (wilson.h:234-312) on the GPU.
+    `gauge` is the Lattice context the gauge field was uploaded to (the operator holds a
+    non-owning reference, as HILA's holds Field<SU> (&gauge)[NDIM])."""
+
+    par = EVEN
+
+    def __init__(self, kappa, gauge: Lattice):
+        self.kappa = float(kappa)
+        self.lat = gauge
+
+    def apply(self, fin: Field, fout: Field):
+        check(lib().wl_meo_apply(self.lat.handle, fout.handle, fin.handle, self.kappa, 1))
+
+    def dagger(self, fin: Field, fout: Field):
+        check(lib().wl_meo_apply(self.lat.handle, fout.handle, fin.handle, self.kappa, -1))
+
+    def hop_set(self, fin: Field, fout: Field, sign=1):
+        """Dirac_Wilson_hop_set: fout = -kappa H_sign fin, fout of the other parity."""
+        check(lib().wl_dslash_eo(self.lat.handle, fout.handle, fin.handle, self.kappa, sign))
+
+    def time(self, fin: Field, fout: Field, what="hop", sign=1, warmup=3, iters=100):
+        """Average device milliseconds per call (HIP events on the library's stream) of the
+        hopping term (what='hop') or of M (what='M')."""
+        ms = C.c_double(0)
+        check(lib().wl_time_op(self.lat.handle, 0 if what == "hop" else 1, fout.handle,
+                               fin.handle, self.kappa, sign, warmup, iters, C.byref(ms)))
+        return ms.value
+
+
+class CG:
+    """HILA's CG<Op> (conjugate_gradient.h:19-105): solves (M^dag M) out = in."""
+
+    def __init__(self, op: Dirac_Wilson_evenodd, accuracy=1e-12, maxiters=10000):
+        self.M = op
+        self.accuracy = float(accuracy)
+        self.maxiters = int(maxiters)
+
+    def apply(self, fin: Field, fout: Field, raise_on_noconv=False):
+        """Runs CG; fout holds the initial guess on entry.  Returns per-rhs stats dicts.
+        Like HILA, reaching maxiters is not an error unless raise_on_noconv is set."""
+        stats = (CGStats * fin.nrhs)()
+        rc = lib().wl_cg_solve(self.M.lat.handle, fout.handle, fin.handle, self.M.kappa,
+                               self.accuracy, self.maxiters, stats)
+        check(rc, allow=() if raise_on_noconv else (WL_E_NOCONV,))
+        return [s.as_dict() for s in stats]

@@ -1,0 +1,169 @@
+/*
+ * wilson_mi355x.h -- C ABI of the MI355X (gfx950) Wilson even-odd Dirac operator and
+ * conjugate-gradient inverter.
+ *
+ * This library replaces one path of HILA (CFT-HY/HILA):
+ *   - Dirac_Wilson_evenodd<SU<3,double>>::apply / ::dagger  (libraries/dirac/wilson.h:272-289)
+ *     together with the hopping kernels it calls, Dirac_Wilson_hop / Dirac_Wilson_hop_set
+ *     (libraries/dirac/wilson.h:15-95);
+ *   - CG<Op>::apply  (libraries/dirac/conjugate_gradient.h:46-104).
+ *
+ * Conventions (identical to HILA's on this path):
+ *   - 4 dimensions, SU(3) colour, directions e_x,e_y,e_z,e_t = 0..3
+ *     (libraries/plumbing/coordinates.h:34-44).
+ *   - Host field buffers are HILA's CPU ("vanilla") AoS layout: one element per site, sites in
+ *     the EVEN_SITES_FIRST order (libraries/plumbing/lattice.cpp:171-186): the lexicographic
+ *     index i = x + Lx*(y + Ly*(z + Lz*t)) (x fastest) maps to i/2 for even sites and
+ *     i/2 + V/2 for odd sites.  A Wilson vector is 24 doubles [spin 0..3][colour 0..2]{re,im}
+ *     (libraries/datatypes/wilson_vector.h:56-63); an SU(3) link is 18 doubles, row-major
+ *     3x3 complex (libraries/datatypes/matrix.h:130).
+ *   - gamma basis of libraries/datatypes/wilson_vector.h:311-347, direction->gamma map
+ *     {g1,g2,g3,g0} (:23), unnormalised projectors (1 +- gamma) (:390-567).
+ *   - D_s psi(x) = psi(x) - kappa sum_mu [ U_mu(x)(1 + s g_mu) psi(x+mu)
+ *                                       + U_mu(x-mu)^dag (1 - s g_mu) psi(x-mu) ],
+ *     s = +1 for apply, -1 for dagger.  The backward hop uses the link and spinor of x-mu;
+ *     HILA's wilson.h:47 reads the forward neighbour's temporary there instead, which makes
+ *     dagger differ from the adjoint of apply.  This library implements the adjoint-consistent
+ *     operator that HILA's own adjointness test (applications/test_cases/test_CG.cpp:236-245)
+ *     requires.
+ *   - M_s = 1 - kappa^2 D_s,eo D_s,oe acting on even sites; the odd half of an
+ *     apply/dagger output is zero (wilson.h:278).
+ *   - Boundary: periodic in x,y,z; antiperiodic in t when requested (psi(x + Lt e_t) = -psi(x)),
+ *     as HILA's SPECIAL_BOUNDARY_CONDITIONS fermion fields (test_CG.cpp:222-226).
+ *
+ * Errors: every function returns WL_OK (0) or a negative WL_E_* code; no exceptions cross
+ * this boundary.  wl_last_error() returns a per-thread message for the last failure.
+ * Threading: one wl_ctx per host thread, or serialise calls on a shared ctx.  All device work
+ * of a ctx is ordered on that ctx's own HIP stream.
+ */
+#ifndef WILSON_MI355X_H
+#define WILSON_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+enum {
+    WL_OK = 0,
+    WL_E_ARG = -1,     /* invalid argument (bad size, null pointer, mismatched field)        */
+    WL_E_OOM = -2,     /* device allocation failed / admission refused by the HBM budget     */
+    WL_E_DEVICE = -3,  /* HIP runtime error                                                  */
+    WL_E_NOCONV = -4,  /* CG reached maxiters without meeting the tolerance                  */
+    WL_E_STATE = -5,   /* call out of order (e.g. no gauge field uploaded)                   */
+    WL_E_NOTIMPL = -6  /* option not provided by this build                                  */
+};
+
+/* Parity, numbered as HILA's enum class Parity {none, even, odd, all}
+ * (libraries/plumbing/coordinates.h:182). */
+enum { WL_EVEN = 1, WL_ODD = 2 };
+
+/* Storage / arithmetic precision of a device field or operator. */
+enum { WL_F64 = 0, WL_F32 = 1, WL_F16 = 2 };
+
+typedef struct wl_ctx wl_ctx;
+typedef struct wl_field wl_field;
+
+/* Statistics of one CG solve (an addition: HILA's CG prints steps, time and the squared
+ * iterated residual ratio, conjugate_gradient.h:98-103, and returns nothing). */
+typedef struct wl_cg_stats {
+    int iterations;          /* CG iterations executed (HILA's printed step count + 1)      */
+    int converged;           /* 1 if ||r||^2 < accuracy^2 ||b||^2 was reached                */
+    double rr_rel;           /* final iterated ||r||^2 / ||b||^2 (HILA's "relative residue") */
+    double true_resid;       /* ||b - M^dag M x|| / ||b||, recomputed in FP64 after the solve */
+    double ms;               /* wall time of the solve on the host clock, milliseconds       */
+    int reliable_updates;    /* mixed-precision solves: number of FP64 reliable updates      */
+    int pad_;
+} wl_cg_stats;
+
+/* ---- library / context ---------------------------------------------------------------- */
+
+/* Version string of this build. */
+const char *wl_version(void);
+/* Message for the last failing call on this host thread ("" if none). */
+const char *wl_last_error(void);
+
+/* Create a context for one L = dims[0..3] = (Lx,Ly,Lz,Lt) lattice on HIP device `device`.
+ * Every L must be even and >= 2 (even-odd decomposition).  Replaces lattice.setup()
+ * (libraries/plumbing/lattice.h:477-480) for this path. */
+int wl_create(const int dims[4], int antiperiodic_t, int device, wl_ctx **out);
+int wl_destroy(wl_ctx *ctx);
+/* Block the host until all work queued on the context's stream has finished. */
+int wl_sync(wl_ctx *ctx);
+/* Number of lattice sites V, and V/2 (sites per parity). */
+int64_t wl_volume(const wl_ctx *ctx);
+
+/* Upload the gauge field.  U[mu] points at HILA's Field<SU<3,double>> gauge[mu] buffer
+ * (V sites x 18 doubles, EVEN_SITES_FIRST order).  Replaces the non-owning gauge reference
+ * that Dirac_Wilson_evenodd holds (wilson.h:237,261). */
+int wl_gauge_upload(wl_ctx *ctx, const double *const U[4]);
+
+/* ---- device fields (one parity, nrhs right-hand sides) -------------------------------- */
+
+/* A device Wilson-vector field on the sites of one parity (WL_EVEN / WL_ODD), holding nrhs
+ * independent vectors, stored in precision prec. */
+int wl_field_create(wl_ctx *ctx, int parity, int prec, int nrhs, wl_field **out);
+int wl_field_destroy(wl_field *f);
+/* Copy rhs vector `rhs` of f from / to host AoS buffer `host` of V/2 sites x 24 doubles, i.e.
+ * the parity block of a HILA Field<WilsonVector<3,double>> buffer: its first V/2 sites for
+ * WL_EVEN, the next V/2 for WL_ODD. */
+int wl_field_upload(wl_field *f, int rhs, const double *host);
+int wl_field_download(const wl_field *f, int rhs, double *host);
+int wl_field_zero(wl_field *f);
+/* sum over sites of ||f||^2 for one rhs, computed on the device in FP64 (the reduction of
+ * conjugate_gradient.h:61,72,79,87). */
+int wl_field_norm2(const wl_field *f, int rhs, double *out);
+
+/* ---- operator ------------------------------------------------------------------------- */
+
+/* Hopping half-operator, HILA's Dirac_Wilson_hop_set(gauge, kappa, in, out, par, sign)
+ * (wilson.h:53-95): out = -kappa * H_s in, with H_s the sum over the 8 hops of
+ * D_s above; out has parity opposite to in.  All nrhs vectors are processed. */
+int wl_dslash_eo(wl_ctx *ctx, wl_field *out, const wl_field *in, double kappa, int sign);
+
+/* out = M_s in on even sites: sign=+1 is Dirac_Wilson_evenodd::apply (wilson.h:272-279),
+ * sign=-1 is ::dagger (wilson.h:282-289).  in and out are even fields. */
+int wl_meo_apply(wl_ctx *ctx, wl_field *out, const wl_field *in, double kappa, int sign);
+
+/* ---- inverter ------------------------------------------------------------------------- */
+
+/* CG on M^dag M, HILA's CG<Dirac_Wilson_evenodd>(D, accuracy, maxiters).apply(b, x)
+ * (conjugate_gradient.h:33-104): x is the initial guess on entry and the solution on exit;
+ * stops when ||r||^2 < accuracy^2 * ||b||^2.  FP64 throughout.  Each of the nrhs vectors is
+ * solved in turn; stats (may be NULL) points to nrhs entries.  Returns WL_E_NOCONV (with x
+ * and stats filled) if maxiters is reached first for any rhs. */
+int wl_cg_solve(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa, double accuracy,
+                int maxiters, wl_cg_stats *stats);
+
+/* ---- diagnostics ---------------------------------------------------------------------- */
+
+/* Average device time per call of `iters` back-to-back calls (after `warmup` untimed ones),
+ * measured with HIP events recorded on the context's stream, around the hopping term
+ * (what = 0, as wl_dslash_eo) or M (what = 1, as wl_meo_apply).  No counterpart in HILA
+ * (its benchmark times the host loop, applications/benchmarks/bench_fermion.cpp:100-113). */
+int wl_time_op(wl_ctx *ctx, int what, wl_field *out, const wl_field *in, double kappa, int sign,
+               int warmup, int iters, double *ms_per_launch);
+
+/* ---- seeded synthetic inputs (host side, counter-based, independent of thread count) --- */
+
+/* Gauge field written in HILA's layout into U[mu] (V x 18 doubles each).
+ * kind 0: unit links; 1: Haar-random SU(3); 2: exp(i*eps*H), H traceless Hermitian with
+ * unit-variance complex Gaussian entries, reunitarized as SU<N>::reunitarize
+ * (libraries/datatypes/sun_matrix.h:137-197). */
+int wl_gen_gauge(const int dims[4], int kind, double eps, uint64_t seed, double *const U[4]);
+/* Gaussian Wilson-vector field: HILA's AoS layout, V x 24 doubles (full lattice, EVEN_SITES_
+ * FIRST); sites of the parities in parity_mask (1 even, 2 odd, 3 all) get re, im ~ N(0,1)
+ * (Complex::gaussian_random, libraries/datatypes/cmplx.h:344-349); the others get 0. */
+int wl_gen_spinor(const int dims[4], uint64_t seed, int parity_mask, double *out);
+/* Point source: 1 at component (spin, colour) of one site chosen from seed, 0 elsewhere
+ * (V x 24 doubles, full lattice).  *site_out receives the EVEN_SITES_FIRST index. */
+int wl_gen_point(const int dims[4], uint64_t seed, int spin, int colour, double *out,
+                 int64_t *site_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WILSON_MI355X_H */

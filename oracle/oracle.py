@@ -1,0 +1,125 @@
+"""ctypes loader for the C restatement of HILA's Wilson operator and CG (wilson_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by bench.py's
+CPU-baseline leg, as the checker.  The product (hila_amd/) never imports it.
+
+Fields are full-lattice numpy arrays in HILA's AoS layout: Wilson vectors (V, 24) float64 and
+links (4, V, 18) float64, sites in EVEN_SITES_FIRST order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libwilson_oracle.so")
+
+EVEN, ODD, ALL = 1, 2, 3
+
+_lib = None
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp, dp, i64 = C.c_void_p, C.c_void_p, C.c_int64
+        L.olat_create.restype = vp
+        L.olat_create.argtypes = [C.POINTER(C.c_int), C.c_int]
+        L.olat_destroy.argtypes = [vp]
+        L.olat_volume.restype = i64
+        L.olat_volume.argtypes = [vp]
+        L.olat_neighbour.restype = i64
+        L.olat_neighbour.argtypes = [vp, C.c_int, i64]
+        for fn in ("oracle_hop", "oracle_hop_set", "oracle_hop_set_fused"):
+            getattr(L, fn).argtypes = [vp, dp, C.c_double, dp, dp, C.c_int, C.c_int]
+        L.oracle_meo.argtypes = [vp, dp, C.c_double, dp, dp, C.c_int]
+        L.oracle_norm2.restype = C.c_double
+        L.oracle_norm2.argtypes = [vp, dp, C.c_int]
+        L.oracle_cg.restype = C.c_int
+        L.oracle_cg.argtypes = [vp, dp, C.c_double, dp, dp, C.c_double, C.c_int,
+                                C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """One lattice geometry of the CPU restatement."""
+
+    def __init__(self, dims, antiperiodic_t=True):
+        self.dims = tuple(int(d) for d in dims)
+        arr = (C.c_int * 4)(*self.dims)
+        self._h = lib().olat_create(arr, 1 if antiperiodic_t else 0)
+        if not self._h:
+            raise ValueError(f"invalid lattice {self.dims}")
+        self.V = int(lib().olat_volume(self._h))
+        self.Vh = self.V // 2
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.olat_destroy(h)
+            self._h = None
+
+    def neighbour(self, d, s):
+        return int(lib().olat_neighbour(self._h, d, s))
+
+    def _field(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.shape == (self.V, 24)
+        return f
+
+    def _links(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == (4, self.V, 18)
+        return U
+
+    def hop_set(self, U, kappa, fin, par, sign, fused=False):
+        """Dirac_Wilson_hop_set: out_par = -kappa H_sign in; other parity left at 0."""
+        U, fin = self._links(U), self._field(fin)
+        out = np.zeros_like(fin)
+        fn = lib().oracle_hop_set_fused if fused else lib().oracle_hop_set
+        fn(self._h, _p(U), kappa, _p(fin), _p(out), par, sign)
+        return out
+
+    def hop(self, U, kappa, fin, fout, par, sign):
+        """Dirac_Wilson_hop: fout_par -= kappa H_sign in (fout updated in place)."""
+        U, fin = self._links(U), self._field(fin)
+        assert fout.dtype == np.float64 and fout.flags.c_contiguous
+        lib().oracle_hop(self._h, _p(U), kappa, _p(fin), _p(fout), par, sign)
+        return fout
+
+    def meo(self, U, kappa, fin, sign):
+        """Dirac_Wilson_evenodd::apply (sign=+1) / dagger (sign=-1) on a full field."""
+        U, fin = self._links(U), self._field(fin)
+        out = np.zeros_like(fin)
+        lib().oracle_meo(self._h, _p(U), kappa, _p(fin), _p(out), sign)
+        return out
+
+    def norm2(self, f, par=EVEN):
+        return float(lib().oracle_norm2(self._h, _p(self._field(f)), par))
+
+    def cg(self, U, kappa, b, x0=None, accuracy=1e-12, maxiters=10000):
+        """CG<Dirac_Wilson_evenodd>(D, accuracy, maxiters).apply(b, x).
+        Returns (x, steps, rr_rel) with HILA's printed step count and relative residue."""
+        U, b = self._links(U), self._field(b)
+        x = np.zeros_like(b) if x0 is None else self._field(x0).copy()
+        rr = C.c_double(0)
+        steps = lib().oracle_cg(self._h, _p(U), kappa, _p(b), _p(x), accuracy, maxiters,
+                                C.byref(rr))
+        return x, int(steps), float(rr.value)
