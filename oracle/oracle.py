@@ -15,24 +15,27 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libwilson_oracle.so")
-
 EVEN, ODD, ALL = 1, 2, 3
 
-_lib = None
+_libs = {}
 
 
-def build():
-    """Compile the oracle with its Makefile (gcc, OpenMP)."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+def _lib_path(variant=0):
+    name = "libwilson_oracle.so" if variant == 0 else f"libwilson_oracle_plant{variant}.so"
+    return os.path.join(_HERE, "_build", name)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def build(force=False):
+    """Compile the oracle with its Makefile (gcc, OpenMP); force rebuilds everything."""
+    subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+
+
+def lib(variant=0):
+    """The oracle library; variant n > 0 is the mutation-test build with planted mistake n."""
+    if variant not in _libs:
+        if not os.path.exists(_lib_path(variant)):
             build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(_lib_path(variant))
         vp, dp, i64 = C.c_void_p, C.c_void_p, C.c_int64
         L.olat_create.restype = vp
         L.olat_create.argtypes = [C.POINTER(C.c_int), C.c_int]
@@ -49,8 +52,8 @@ def lib():
         L.oracle_cg.restype = C.c_int
         L.oracle_cg.argtypes = [vp, dp, C.c_double, dp, dp, C.c_double, C.c_int,
                                 C.POINTER(C.c_double)]
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def _p(a):
@@ -61,23 +64,24 @@ def _p(a):
 class Oracle:
     """One lattice geometry of the CPU restatement."""
 
-    def __init__(self, dims, antiperiodic_t=True):
+    def __init__(self, dims, antiperiodic_t=True, variant=0):
         self.dims = tuple(int(d) for d in dims)
+        self._L = lib(variant)
         arr = (C.c_int * 4)(*self.dims)
-        self._h = lib().olat_create(arr, 1 if antiperiodic_t else 0)
+        self._h = self._L.olat_create(arr, 1 if antiperiodic_t else 0)
         if not self._h:
             raise ValueError(f"invalid lattice {self.dims}")
-        self.V = int(lib().olat_volume(self._h))
+        self.V = int(self._L.olat_volume(self._h))
         self.Vh = self.V // 2
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h and _lib is not None:
-            _lib.olat_destroy(h)
+        if h:
+            self._L.olat_destroy(h)
             self._h = None
 
     def neighbour(self, d, s):
-        return int(lib().olat_neighbour(self._h, d, s))
+        return int(self._L.olat_neighbour(self._h, d, s))
 
     def _field(self, f):
         f = np.ascontiguousarray(f, dtype=np.float64)
@@ -93,7 +97,7 @@ class Oracle:
         """Dirac_Wilson_hop_set: out_par = -kappa H_sign in; other parity left at 0."""
         U, fin = self._links(U), self._field(fin)
         out = np.zeros_like(fin)
-        fn = lib().oracle_hop_set_fused if fused else lib().oracle_hop_set
+        fn = self._L.oracle_hop_set_fused if fused else self._L.oracle_hop_set
         fn(self._h, _p(U), kappa, _p(fin), _p(out), par, sign)
         return out
 
@@ -101,18 +105,18 @@ class Oracle:
         """Dirac_Wilson_hop: fout_par -= kappa H_sign in (fout updated in place)."""
         U, fin = self._links(U), self._field(fin)
         assert fout.dtype == np.float64 and fout.flags.c_contiguous
-        lib().oracle_hop(self._h, _p(U), kappa, _p(fin), _p(fout), par, sign)
+        self._L.oracle_hop(self._h, _p(U), kappa, _p(fin), _p(fout), par, sign)
         return fout
 
     def meo(self, U, kappa, fin, sign):
         """Dirac_Wilson_evenodd::apply (sign=+1) / dagger (sign=-1) on a full field."""
         U, fin = self._links(U), self._field(fin)
         out = np.zeros_like(fin)
-        lib().oracle_meo(self._h, _p(U), kappa, _p(fin), _p(out), sign)
+        self._L.oracle_meo(self._h, _p(U), kappa, _p(fin), _p(out), sign)
         return out
 
     def norm2(self, f, par=EVEN):
-        return float(lib().oracle_norm2(self._h, _p(self._field(f)), par))
+        return float(self._L.oracle_norm2(self._h, _p(self._field(f)), par))
 
     def cg(self, U, kappa, b, x0=None, accuracy=1e-12, maxiters=10000):
         """CG<Dirac_Wilson_evenodd>(D, accuracy, maxiters).apply(b, x).
@@ -120,6 +124,6 @@ class Oracle:
         U, b = self._links(U), self._field(b)
         x = np.zeros_like(b) if x0 is None else self._field(x0).copy()
         rr = C.c_double(0)
-        steps = lib().oracle_cg(self._h, _p(U), kappa, _p(b), _p(x), accuracy, maxiters,
+        steps = self._L.oracle_cg(self._h, _p(U), kappa, _p(b), _p(x), accuracy, maxiters,
                                 C.byref(rr))
         return x, int(steps), float(rr.value)

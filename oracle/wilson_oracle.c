@@ -42,6 +42,15 @@
 #define NDIRS 8
 #define NC 3
 
+/* Mutation-test builds only: -DORACLE_PLANT=n compiles one deliberate mistake into the
+ * oracle so that the tests can show their pins reject it (tests/test_oracle_mutations.py).
+ * 1: the backward hop as written in wilson.h:47 (reads vtemp[dir][X+dir]);
+ * 2: no antiperiodic sign;  3: sign error in the e_z projection;
+ * 4: U instead of U^dag on the backward hop;  5: gamma_t rows swapped in expand. */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+
 /* Parity as HILA's enum class Parity {none, even, odd, all} (coordinates.h:182). */
 enum { P_NONE = 0, P_EVEN = 1, P_ODD = 2, P_ALL = 3 };
 
@@ -140,7 +149,7 @@ olat *olat_create(const int L[4], int antiperiodic_t) {
                 }
             }
             l->nb[d][s] = site_index(l, y);
-            l->sg[d][s] = (cross && mu == 3 && antiperiodic_t) ? -1 : 1;
+            l->sg[d][s] = (cross && mu == 3 && antiperiodic_t && ORACLE_PLANT != 2) ? -1 : 1;
         }
     }
     return l;
@@ -185,7 +194,7 @@ static hvec half_project(const wvec *w, int dir, int sign) {
             h.c[1][a] = sign == 1 ? cadd(w1, w2) : csub(w1, w2);
             break;
         case 2: /* e_z */
-            h.c[0][a] = sign == 1 ? cadd(w0, ci(w2)) : csub(w0, ci(w2));
+            h.c[0][a] = (sign == 1) != (ORACLE_PLANT == 3) ? cadd(w0, ci(w2)) : csub(w0, ci(w2));
             h.c[1][a] = sign == 1 ? csub(w1, ci(w3)) : cadd(w1, ci(w3));
             break;
         default: /* e_t */
@@ -220,6 +229,11 @@ static wvec half_expand(const hvec *h, int dir, int sign) {
         default:
             r.c[2][a] = sign == 1 ? c0 : cneg(c0);
             r.c[3][a] = sign == 1 ? c1 : cneg(c1);
+            if (ORACLE_PLANT == 5) {
+                const cplx tmp = r.c[2][a];
+                r.c[2][a] = r.c[3][a];
+                r.c[3][a] = tmp;
+            }
             break;
         }
     }
@@ -300,7 +314,7 @@ static void hop_project(const olat *l, const double *U, const double *in, int pa
         for (int64_t s = 0; s < l->V; ++s) {
             if (!in_parity(l, s, opp)) continue;
             hvec h = half_project(W(in, s), dir, -sign);
-            su3 ua = su3_adjoint(Ulink(U, l->V, dir, s));
+            su3 ua = ORACLE_PLANT == 4 ? *Ulink(U, l->V, dir, s) : su3_adjoint(Ulink(U, l->V, dir, s));
             vt->v[NDIRS - 1 - dir][s] = su3_mul_h(&ua, &h); /* vtemp[-dir][X] */
             vt->v[dir][s] = half_project(W(in, s), dir, sign); /* vtemp[dir][X] */
         }
@@ -318,7 +332,8 @@ static void hop_accumulate_dir(const olat *l, const double *U, double kappa, con
         if (!in_parity(l, s, par)) continue;
         const int64_t sf = l->nb[dir][s], sb = l->nb[bdir][s];
         const hvec hf = h_scale((double)l->sg[dir][s], &vt->v[dir][sf]);
-        const hvec hb = h_scale((double)l->sg[bdir][s], &vt->v[bdir][sb]);
+        const hvec hb = ORACLE_PLANT == 1 ? h_scale((double)l->sg[dir][s], &vt->v[dir][sf])
+                                          : h_scale((double)l->sg[bdir][s], &vt->v[bdir][sb]);
         const su3 ku = su3_scale(kappa, Ulink(U, l->V, dir, s));
         const hvec uf = su3_mul_h(&ku, &hf);
         const hvec kb = h_scale(kappa, &hb);
