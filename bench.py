@@ -67,6 +67,10 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cg-solves", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: --steps hopping launches, then --steps "
+                         "plain field copies (12 planes of V/2 complex doubles each way) as "
+                         "the byte-counter calibration; prints no JSON line")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -85,6 +89,18 @@ def main(argv=None):
     fodd = W.Field(lat, W.ODD)
     vh = lat.Vh
     print(f"# setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
+
+    if a.profile:
+        fcopy = W.Field(lat, W.EVEN)
+        for _ in range(a.steps):
+            D.hop_set(fin, fodd, 1)
+        lat.sync()
+        for _ in range(a.steps):
+            fcopy.copy_from(fin)
+        lat.sync()
+        print(f"# profile: {a.steps} hopping launches, {a.steps} copies of "
+              f"{12 * 16 * vh} bytes", file=sys.stderr, flush=True)
+        return
 
     # --- Dslash: W warm-up, K timed steps, synchronised on both sides ---
     for _ in range(a.warmup):

@@ -184,7 +184,8 @@ extern "C" int wl_create(const int dims[4], int antiperiodic_t, int device, wl_c
     c->g.antiperiodic = c->antiperiodic;
     DeviceGuard dg(device);
     int rc = WL_OK;
-    const int nb = nblocks(c->g);
+    // one partial-sum slot per workgroup of the largest grid (64-lane workgroups)
+    const int nb = (c->g.vh + 63) / 64;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->partials, sizeof(double) * (nb + 64)) != hipSuccess ||
         hipMalloc(&c->ticket, 256) != hipSuccess ||
@@ -536,5 +537,19 @@ extern "C" int wl_time_op(wl_ctx *c, int what, wl_field *out, const wl_field *in
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms_per_launch = (double)ms / iters;
+    return WL_OK;
+}
+
+extern "C" int wl_field_copy(wl_field *dst, const wl_field *src) {
+    if (!dst || !src) return fail(WL_E_ARG, "null argument");
+    if (dst->ctx != src->ctx) return fail(WL_E_ARG, "fields belong to different contexts");
+    if (dst->parity != src->parity || dst->prec != src->prec || dst->nrhs != src->nrhs)
+        return fail(WL_E_ARG, "fields differ in parity, precision or nrhs");
+    if (dst == src) return WL_OK;
+    wl_ctx *c = dst->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    for (int r = 0; r < src->nrhs; ++r)
+        HIPCHK(launch_copy(src->prec, c->g, src->rhs_ptr(r), dst->rhs_ptr(r), c->stream));
     return WL_OK;
 }

@@ -71,6 +71,30 @@ __device__ __forceinline__ R norm2(cx<R> a) {
     return a.x * a.x + a.y * a.y;
 }
 
+// Loads / stores of one complex number.  NT = nontemporal: streamed data that this kernel does
+// not read again (links, the output), so that it does not displace the neighbour spinors that
+// the caches are meant to keep.
+template <bool NT, typename R>
+__device__ __forceinline__ cx<R> ldc(const cx<R> *p) {
+    if constexpr (NT) {
+        typedef R v2 __attribute__((ext_vector_type(2)));
+        const v2 v = __builtin_nontemporal_load((const v2 *)p);
+        return {v.x, v.y};
+    } else {
+        return *p;
+    }
+}
+template <bool NT, typename R>
+__device__ __forceinline__ void stc(cx<R> *p, cx<R> a) {
+    if constexpr (NT) {
+        typedef R v2 __attribute__((ext_vector_type(2)));
+        const v2 v = {a.x, a.y};
+        __builtin_nontemporal_store(v, (v2 *)p);
+    } else {
+        *p = a;
+    }
+}
+
 template <typename R>
 using Spinor = cx<R>[4][3];
 template <typename R>
@@ -219,8 +243,8 @@ __device__ __forceinline__ double block_sum(double v) {
     __syncthreads();
     double s = 0;
     if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < kBlock / 64; ++i) s += part[i];
+        const int nw = (int)(blockDim.x >> 6);
+        for (int i = 0; i < nw; ++i) s += part[i];
     }
     __syncthreads();
     return s;

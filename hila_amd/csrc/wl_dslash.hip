@@ -10,6 +10,12 @@
 // on the two t hops that cross the edge (HILA fills negated halo copies instead,
 // backend_gpu/field_storage_backend.h:333-384).
 //
+// The 8 hops are software-pipelined: the link and neighbour spinor of hop h+1 are requested
+// before hop h is computed, so every wave keeps 15-25 16-byte loads in flight (the kernel is
+// bound by HBM bandwidth and latency, not arithmetic: 0.86 flop/byte in FP64).  Links and the
+// output are streamed with nontemporal accesses; neighbour spinors go through the caches, where
+// their 8-fold reuse is caught.
+//
 // Epilogues fuse the even-odd operator and CG vector updates (wilson.h:272-289,
 // conjugate_gradient.h:77-94) into the same pass; see Epi in wl_internal.h.
 
@@ -34,65 +40,74 @@ struct DslashArgs {
     int fin;
 };
 
+// Registers holding one hop's operands: neighbour spinor and link.
 template <typename R>
-__device__ __forceinline__ void load_spinor(const cx<R> *__restrict__ f, int S, int k,
-                                            cx<R> (&w)[4][3]) {
+struct HopBuf {
+    cx<R> w[4][3];
+    cx<R> u[3][3];
+};
+
+template <typename R>
+__device__ __forceinline__ void hop_load(HopBuf<R> &b, const cx<R> *__restrict__ in,
+                                         const cx<R> *__restrict__ link, int S, int kn, int kl) {
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) w[s][c] = f[(s * 3 + c) * S + k];
-}
-
-template <typename R>
-__device__ __forceinline__ void load_link(const cx<R> *__restrict__ u0, int S, int k,
-                                          cx<R> (&u)[3][3]) {
+        for (int c = 0; c < 3; ++c) b.w[s][c] = in[(s * 3 + c) * S + kn];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) u[i][j] = u0[(i * 3 + j) * S + k];
+        for (int j = 0; j < 3; ++j) b.u[i][j] = ldc<true>(link + (i * 3 + j) * S + kl);
 }
 
-// Both hops along direction MU.  BC: multiply by sf / sb (only the t direction can cross an
-// antiperiodic edge).
-template <int MU, int SIGN, bool BC, typename R>
-__device__ __forceinline__ void hop_dir(cx<R> (&acc)[4][3], const cx<R> *__restrict__ in,
-                                        const cx<R> *__restrict__ gout,
-                                        const cx<R> *__restrict__ gin, int S, int k, int kf,
-                                        int kb, R sf, R sb) {
-    {
-        cx<R> w[4][3], h[2][3], u[3][3], v[2][3];
-        load_spinor(in, S, kf, w);
-        load_link(gout + MU * 9 * S, S, k, u);
-        project<MU, SIGN>(w, h);
-        su3_mul(u, h, v);
-        if constexpr (BC) {
-#pragma unroll
-            for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) v[q][c] = sf * v[q][c];
-        }
-        expand_add<MU, SIGN>(acc, v);
+// acc += expand(U (1 + s g_mu) w) for a forward hop, expand(U^dag (1 - s g_mu) w) backward;
+// sg = -1 for a t hop across the antiperiodic edge.
+template <int MU, int SIGN, bool FWD, typename R>
+__device__ __forceinline__ void hop_compute(cx<R> (&acc)[4][3], const HopBuf<R> &b, R sg) {
+    cx<R> h[2][3], v[2][3];
+    if constexpr (FWD) {
+        project<MU, SIGN>(b.w, h);
+        su3_mul(b.u, h, v);
+    } else {
+        project<MU, -SIGN>(b.w, h);
+        su3_adj_mul(b.u, h, v);
     }
-    {
-        cx<R> w[4][3], h[2][3], u[3][3], v[2][3];
-        load_spinor(in, S, kb, w);
-        load_link(gin + MU * 9 * S, S, kb, u);
-        project<MU, -SIGN>(w, h);
-        su3_adj_mul(u, h, v);
-        if constexpr (BC) {
+    if constexpr (MU == 3) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q)
+        for (int q = 0; q < 2; ++q)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) v[q][c] = sb * v[q][c];
-        }
+            for (int c = 0; c < 3; ++c) v[q][c] = sg * v[q][c];
+    }
+    if constexpr (FWD)
+        expand_add<MU, SIGN>(acc, v);
+    else
         expand_add<MU, -SIGN>(acc, v);
+}
+
+// Hop H = 2 mu + (0 forward, 1 backward): issue hop H+1's loads, then compute hop H.
+template <typename R, int SIGN, int H>
+__device__ __forceinline__ void hop_chain(cx<R> (&acc)[4][3], HopBuf<R> (&buf)[2],
+                                          const cx<R> *__restrict__ in, const cx<R> *gout,
+                                          const cx<R> *gin, int S, int k, const SiteNb &n, R sf,
+                                          R sb) {
+    if constexpr (H < 8) {
+        constexpr int HN = H + 1;
+        if constexpr (HN < 8) {
+            constexpr int MN = HN / 2;
+            if constexpr (HN % 2 == 0)
+                hop_load(buf[HN & 1], in, gout + MN * 9 * S, S, n.fwd[MN], k);
+            else
+                hop_load(buf[HN & 1], in, gin + MN * 9 * S, S, n.bwd[MN], n.bwd[MN]);
+        }
+        hop_compute<H / 2, SIGN, (H % 2 == 0)>(acc, buf[H & 1], (H % 2 == 0) ? sf : sb);
+        hop_chain<R, SIGN, H + 1>(acc, buf, in, gout, gin, S, k, n, sf, sb);
     }
 }
 
 template <typename R, int SIGN, int EPI>
 __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
     if (A.done && *A.done) return;
-    const int k = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int S = A.g.stride;
     double nrm = 0;
     if (k < A.g.vh) {
@@ -106,25 +121,24 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
         for (int s = 0; s < 4; ++s)
 #pragma unroll
             for (int c = 0; c < 3; ++c) acc[s][c] = {R(0), R(0)};
-        hop_dir<0, SIGN, false>(acc, A.in, gout, gin, S, k, n.fwd[0], n.bwd[0], R(1), R(1));
-        hop_dir<1, SIGN, false>(acc, A.in, gout, gin, S, k, n.fwd[1], n.bwd[1], R(1), R(1));
-        hop_dir<2, SIGN, false>(acc, A.in, gout, gin, S, k, n.fwd[2], n.bwd[2], R(1), R(1));
-        hop_dir<3, SIGN, true>(acc, A.in, gout, gin, S, k, n.fwd[3], n.bwd[3], sf, sb);
+        HopBuf<R> buf[2];
+        hop_load(buf[0], A.in, gout, S, n.fwd[0], k);
+        hop_chain<R, SIGN, 0>(acc, buf, A.in, gout, gin, S, k, n, sf, sb);
 
         const R a = A.a;
         if constexpr (EPI == EPI_SET) {
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) A.out[(s * 3 + c) * S + k] = a * acc[s][c];
+                for (int c = 0; c < 3; ++c) stc<true>(A.out + (s * 3 + c) * S + k, a * acc[s][c]);
         } else if constexpr (EPI == EPI_AXPY || EPI == EPI_AXPY_NORM) {
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const int e = (s * 3 + c) * S + k;
-                    const cx<R> o = A.xin[e] + a * acc[s][c];
-                    A.out[e] = o;
+                    const cx<R> o = ldc<true>(A.xin + e) + a * acc[s][c];
+                    stc<true>(A.out + e, o);
                     if constexpr (EPI == EPI_AXPY_NORM) nrm += (double)norm2(o);
                 }
         } else {  // EPI_CGUPD, FP64
@@ -134,15 +148,23 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const int e = (s * 3 + c) * S + k;
-                    const cx<double> ddp = A.xin[e] + a * acc[s][c];
-                    A.x[e] = A.x[e] + alpha * A.p[e];
-                    const cx<double> rn = A.r[e] - alpha * ddp;
-                    A.r[e] = rn;
+                    const cx<double> ddp = ldc<true>(A.xin + e) + a * acc[s][c];
+                    stc<true>(A.x + e, ldc<true>(A.x + e) + alpha * ldc<true>(A.p + e));
+                    const cx<double> rn = ldc<true>(A.r + e) - alpha * ddp;
+                    stc<true>(A.r + e, rn);
                     nrm += norm2(rn);
                 }
         }
     }
     if constexpr (EPI == EPI_AXPY_NORM || EPI == EPI_CGUPD) reduce_finish(nrm, A.fin, A.red);
+}
+
+// Workgroup size: 256 lanes when that still gives >= 4 workgroups per CU on the 256 CUs,
+// otherwise smaller workgroups so that small lattices still spread over all CUs.
+int dslash_block(const Geom &g) {
+    if (g.vh >= 256 * 1024) return 256;
+    if (g.vh >= 128 * 256) return 128;
+    return 64;
 }
 
 int nblocks(const Geom &g) { return (g.vh + kBlock - 1) / kBlock; }
@@ -163,7 +185,9 @@ static hipError_t launch_t(const Geom &g, const DslashLaunch &L, hipStream_t s) 
     A.r = (cx<double> *)L.r;
     A.red = L.red;
     A.fin = L.fin;
-    hipLaunchKernelGGL((k_dslash<R, SIGN, EPI>), dim3(nblocks(g)), dim3(kBlock), 0, s, A);
+    const int blk = dslash_block(g);
+    const int nb = (g.vh + blk - 1) / blk;
+    hipLaunchKernelGGL((k_dslash<R, SIGN, EPI>), dim3(nb), dim3(blk), 0, s, A);
     return hipGetLastError();
 }
 

@@ -31,7 +31,7 @@ F64, F32, F16 = 0, 1, 2
 EXPORTED_SYMBOLS = [
     "wl_version", "wl_last_error", "wl_create", "wl_destroy", "wl_sync", "wl_volume",
     "wl_gauge_upload", "wl_field_create", "wl_field_destroy", "wl_field_upload",
-    "wl_field_download", "wl_field_zero", "wl_field_norm2", "wl_dslash_eo", "wl_meo_apply",
+    "wl_field_download", "wl_field_zero", "wl_field_copy", "wl_field_norm2", "wl_dslash_eo", "wl_meo_apply",
     "wl_cg_solve", "wl_time_op", "wl_gen_gauge", "wl_gen_spinor", "wl_gen_point",
 ]
 
@@ -76,6 +76,7 @@ def lib():
         L.wl_field_upload.argtypes = [vp, i, vp]
         L.wl_field_download.argtypes = [vp, i, vp]
         L.wl_field_zero.argtypes = [vp]
+        L.wl_field_copy.argtypes = [vp, vp]
         L.wl_field_norm2.argtypes = [vp, i, C.POINTER(d)]
         L.wl_dslash_eo.argtypes = [vp, vp, vp, d, i]
         L.wl_meo_apply.argtypes = [vp, vp, vp, d, i]
@@ -234,6 +235,11 @@ class Field:
 
     def zero(self):
         check(lib().wl_field_zero(self._h))
+        return self
+
+    def copy_from(self, src: "Field"):
+        """self = src on the device (HILA's field assignment)."""
+        check(lib().wl_field_copy(self._h, src.handle))
         return self
 
     def norm2(self, rhs=0):

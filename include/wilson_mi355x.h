@@ -113,6 +113,9 @@ int wl_field_destroy(wl_field *f);
 int wl_field_upload(wl_field *f, int rhs, const double *host);
 int wl_field_download(const wl_field *f, int rhs, double *host);
 int wl_field_zero(wl_field *f);
+/* dst = src (same parity, precision and nrhs): HILA's Field assignment dst[par] = src[X]
+ * (libraries/plumbing/field.h:898-909), on the device. */
+int wl_field_copy(wl_field *dst, const wl_field *src);
 /* sum over sites of ||f||^2 for one rhs, computed on the device in FP64 (the reduction of
  * conjugate_gradient.h:61,72,79,87). */
 int wl_field_norm2(const wl_field *f, int rhs, double *out);
