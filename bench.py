@@ -98,6 +98,13 @@ def main(argv=None):
         for _ in range(a.steps):
             fcopy.copy_from(fin)
         lat.sync()
+        if a.cg_solves > 0:
+            fM = W.Field(lat, W.EVEN)
+            for _ in range(5):
+                D.apply(fin, fM)
+            st = W.CG(D, 1e-10, 10000).apply(fin, fM.zero())[0]
+            lat.sync()
+            print(f"# profile: CG {st['iterations']} iterations", file=sys.stderr, flush=True)
         print(f"# profile: {a.steps} hopping launches, {a.steps} copies of "
               f"{12 * 16 * vh} bytes", file=sys.stderr, flush=True)
         return

@@ -23,6 +23,9 @@ LIBDIR = os.path.join(ROOT, "hila_amd", "lib")
 LIB = os.path.join(LIBDIR, "libhila_wilson.so")
 
 HIP_SOURCES = ["wl_dslash.hip", "wl_blas.hip", "wl_api.hip"]
+# C++ programs written against the HILA-style mirror (include/hila_wilson.hpp)
+CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "test_wilson_evenodd.cpp")]
+CPP_TEST_BUILD = os.path.join(ROOT, "tests", "cpp", "_build")
 HOST_SOURCES = ["wl_gen.cpp"]
 
 ARCH = "gfx950"
@@ -74,6 +77,11 @@ def build(resources=False, verbose=False):
     _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
           "-lgomp", "-Wl,--no-undefined"])
     os.replace(tmp, LIB)
+    os.makedirs(CPP_TEST_BUILD, exist_ok=True)
+    for src in CPP_TESTS:
+        exe = os.path.join(CPP_TEST_BUILD, os.path.basename(src)[:-4])
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, src, "-o", exe, "-L", LIBDIR,
+              "-lhila_wilson", "-Wl,-rpath," + LIBDIR + ",-rpath,$ORIGIN/../../../hila_amd/lib"])
     return LIB
 
 

@@ -97,7 +97,6 @@ int ensure_tmp(wl_ctx *c, int prec, wl_field **slot, int parity) {
 RedArgs red_args(wl_ctx *c) {
     RedArgs ra;
     ra.partials = c->partials;
-    ra.ticket = c->ticket;
     ra.st = c->st;
     ra.out = c->scratch_d;
     return ra;
@@ -153,6 +152,37 @@ int meo_one(wl_ctx *c, int prec, void *out, const void *in, void *tmp, double ka
 
 int check_sign(int sign) { return (sign == 1 || sign == -1) ? WL_OK : fail(WL_E_ARG, "sign must be +-1"); }
 
+// Segments (64 checkerboard sites) per time slice, or 0 when a slice is not a whole number of
+// segments (then only the memory-order traversal is available).
+int segments_per_slice(const Geom &g) {
+    const int64_t slice = (int64_t)g.L[0] * g.L[1] * g.L[2] / 2;
+    return (slice % 64 == 0) ? (int)(slice / 64) : 0;
+}
+
+// Default traversal chunk, from measurements on MI355X: one chunk per XCD (a chunk is a
+// slab of z-planes, swept over all t) when the slice splits 8 ways, else the largest divisor
+// of the segments per slice up to a quarter slice.  Lattices below 2048 segments per parity
+// are latency-bound and keep memory order.
+int auto_chunk(const Geom &g) {
+    const int sps = segments_per_slice(g);
+    if (sps == 0 || g.vh / 64 < 2048) return 0;
+    if (sps % 8 == 0) return sps / 8;
+    for (int q = sps / 4; q >= 1; --q)
+        if (sps % q == 0) return q;
+    return 0;
+}
+
+int set_chunk(Geom &g, int qc) {
+    const int sps = segments_per_slice(g);
+    if (qc < 0) qc = auto_chunk(g);
+    if (qc > 0 && (sps == 0 || sps % qc != 0))
+        return fail(WL_E_ARG, "traversal chunk %d must divide the %d segments per time slice", qc, sps);
+    g.qc = qc;
+    g.sps = sps;
+    g.ns = g.vh / 64;
+    return WL_OK;
+}
+
 }  // namespace
 
 // =================================================================== library ============
@@ -182,20 +212,20 @@ extern "C" int wl_create(const int dims[4], int antiperiodic_t, int device, wl_c
     c->g.vh = (int)(V / 2);
     c->g.stride = (int)((V / 2 + kBlock - 1) / kBlock * kBlock);
     c->g.antiperiodic = c->antiperiodic;
+    c->g.blk = dslash_block_auto(c->g.vh);
+    (void)set_chunk(c->g, -1);
     DeviceGuard dg(device);
     int rc = WL_OK;
     // one partial-sum slot per workgroup of the largest grid (64-lane workgroups)
     const int nb = (c->g.vh + 63) / 64;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->partials, sizeof(double) * (nb + 64)) != hipSuccess ||
-        hipMalloc(&c->ticket, 256) != hipSuccess ||
         hipMalloc(&c->st, sizeof(CgState)) != hipSuccess ||
         hipMalloc(&c->scratch_d, 256) != hipSuccess ||
         hipHostMalloc(&c->st_host, sizeof(CgState), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&c->scratch_h, 256, hipHostMallocDefault) != hipSuccess) {
         rc = fail(WL_E_DEVICE, "context allocation failed");
-    } else if (hipMemsetAsync(c->ticket, 0, 256, c->stream) != hipSuccess ||
-               hipMemsetAsync(c->st, 0, sizeof(CgState), c->stream) != hipSuccess ||
+    } else if (hipMemsetAsync(c->st, 0, sizeof(CgState), c->stream) != hipSuccess ||
                hipStreamSynchronize(c->stream) != hipSuccess) {
         rc = fail(WL_E_DEVICE, "context initialisation failed");
     }
@@ -213,8 +243,15 @@ extern "C" int wl_destroy(wl_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (wl_field *f : {c->tmp_odd64, c->tmp_odd32, c->cg_r, c->cg_p, c->cg_Dp, c->cg_DDp})
         field_free(f);
+    // fields the caller still holds: release their device memory and orphan the handles
+    for (wl_field *f : c->user_fields) {
+        if (f->data) (void)hipFree(f->data);
+        f->data = nullptr;
+        f->ctx = nullptr;
+    }
+    c->user_fields.clear();
     for (void *p : {(void *)c->gauge64, c->gauge32, c->staging, (void *)c->partials,
-                    (void *)c->ticket, (void *)c->st, (void *)c->scratch_d})
+                    (void *)c->st, (void *)c->scratch_d})
         if (p) (void)hipFree(p);
     if (c->st_host) (void)hipHostFree(c->st_host);
     if (c->scratch_h) (void)hipHostFree(c->scratch_h);
@@ -265,19 +302,29 @@ extern "C" int wl_field_create(wl_ctx *c, int parity, int prec, int nrhs, wl_fie
     if (nrhs < 1 || nrhs > 64) return fail(WL_E_ARG, "nrhs %d out of range 1..64", nrhs);
     std::lock_guard<std::mutex> lk(c->mu);
     DeviceGuard dg(c->device);
-    return field_new(c, parity, prec, nrhs, out);
+    int rc = field_new(c, parity, prec, nrhs, out);
+    if (rc == WL_OK) c->user_fields.insert(*out);
+    return rc;
 }
 
 extern "C" int wl_field_destroy(wl_field *f) {
     if (!f) return WL_OK;
-    DeviceGuard dg(f->ctx->device);
-    (void)hipStreamSynchronize(f->ctx->stream);
+    wl_ctx *c = f->ctx;
+    if (!c) {  // orphaned by wl_destroy: only the handle is left
+        delete f;
+        return WL_OK;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->user_fields.erase(f);
     field_free(f);
     return WL_OK;
 }
 
 extern "C" int wl_field_upload(wl_field *f, int rhs, const double *host) {
     if (!f || !host) return fail(WL_E_ARG, "null argument");
+    if (!f->ctx) return fail(WL_E_STATE, "the field's context was destroyed");
     if (rhs < 0 || rhs >= f->nrhs) return fail(WL_E_ARG, "rhs %d of %d", rhs, f->nrhs);
     wl_ctx *c = f->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -293,6 +340,7 @@ extern "C" int wl_field_upload(wl_field *f, int rhs, const double *host) {
 
 extern "C" int wl_field_download(const wl_field *f, int rhs, double *host) {
     if (!f || !host) return fail(WL_E_ARG, "null argument");
+    if (!f->ctx) return fail(WL_E_STATE, "the field's context was destroyed");
     if (rhs < 0 || rhs >= f->nrhs) return fail(WL_E_ARG, "rhs %d of %d", rhs, f->nrhs);
     wl_ctx *c = f->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -308,6 +356,7 @@ extern "C" int wl_field_download(const wl_field *f, int rhs, double *host) {
 
 extern "C" int wl_field_zero(wl_field *f) {
     if (!f) return fail(WL_E_ARG, "null argument");
+    if (!f->ctx) return fail(WL_E_STATE, "the field's context was destroyed");
     DeviceGuard dg(f->ctx->device);
     HIPCHK(hipMemsetAsync(f->data, 0, f->bytes, f->ctx->stream));
     return WL_OK;
@@ -315,6 +364,7 @@ extern "C" int wl_field_zero(wl_field *f) {
 
 extern "C" int wl_field_norm2(const wl_field *f, int rhs, double *out) {
     if (!f || !out) return fail(WL_E_ARG, "null argument");
+    if (!f->ctx) return fail(WL_E_STATE, "the field's context was destroyed");
     if (rhs < 0 || rhs >= f->nrhs) return fail(WL_E_ARG, "rhs %d of %d", rhs, f->nrhs);
     wl_ctx *c = f->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -542,6 +592,7 @@ extern "C" int wl_time_op(wl_ctx *c, int what, wl_field *out, const wl_field *in
 
 extern "C" int wl_field_copy(wl_field *dst, const wl_field *src) {
     if (!dst || !src) return fail(WL_E_ARG, "null argument");
+    if (!dst->ctx || !src->ctx) return fail(WL_E_STATE, "the field's context was destroyed");
     if (dst->ctx != src->ctx) return fail(WL_E_ARG, "fields belong to different contexts");
     if (dst->parity != src->parity || dst->prec != src->prec || dst->nrhs != src->nrhs)
         return fail(WL_E_ARG, "fields differ in parity, precision or nrhs");
@@ -552,4 +603,28 @@ extern "C" int wl_field_copy(wl_field *dst, const wl_field *src) {
     for (int r = 0; r < src->nrhs; ++r)
         HIPCHK(launch_copy(src->prec, c->g, src->rhs_ptr(r), dst->rhs_ptr(r), c->stream));
     return WL_OK;
+}
+
+extern "C" int wl_set_option(wl_ctx *c, int option, int value) {
+    if (!c) return fail(WL_E_ARG, "null context");
+    std::lock_guard<std::mutex> lk(c->mu);
+    switch (option) {
+    case WL_OPT_TRAVERSAL_CHUNK: return set_chunk(c->g, value);
+    case WL_OPT_BLOCK:
+        if (value == 0) value = dslash_block_auto(c->g.vh);
+        if (value != 64 && value != 128 && value != 256)
+            return fail(WL_E_ARG, "workgroup size must be 64, 128 or 256");
+        c->g.blk = value;
+        return WL_OK;
+    default: return fail(WL_E_ARG, "unknown option %d", option);
+    }
+}
+
+extern "C" int wl_get_option(const wl_ctx *c, int option, int *value) {
+    if (!c || !value) return fail(WL_E_ARG, "null argument");
+    switch (option) {
+    case WL_OPT_TRAVERSAL_CHUNK: *value = c->g.qc; return WL_OK;
+    case WL_OPT_BLOCK: *value = c->g.blk; return WL_OK;
+    default: return fail(WL_E_ARG, "unknown option %d", option);
+    }
 }

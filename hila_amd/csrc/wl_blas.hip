@@ -2,8 +2,9 @@
 //
 // The reductions replace HILA's per-workgroup shared-memory tree followed by host-side
 // summation of the partials (hilapp codegen_gpu.cpp:1033-1088, backend_gpu/
-// gpu_templated_ops.h:130-162): partials are summed on the device by the last workgroup, in
-// workgroup order, and the result goes straight into the device-resident CG state.
+// gpu_templated_ops.h:130-162): the partials are summed on the device by a one-workgroup
+// finish kernel, in workgroup order, and the result goes straight into the device-resident CG
+// state.
 
 #include "wl_internal.h"
 
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(kBlock) void k_norm2(const cx<R> *__restrict__ f, i
 #pragma unroll
         for (int c = 0; c < 12; ++c) s += (double)norm2(f[c * S + k]);
     }
-    reduce_finish(s, fin, ra);
+    reduce_partial(s, ra);
 }
 
 __global__ __launch_bounds__(kBlock) void k_cg_init(const cx<double> *__restrict__ b,
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(const cx<double> *__restrict
             s += norm2(v);
         }
     }
-    reduce_finish(s, FIN_RR, ra);
+    reduce_partial(s, ra);
 }
 
 __global__ __launch_bounds__(kBlock) void k_cg_xpay(cx<double> *__restrict__ p,
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_norm2(const cx<double> *__restr
             s += norm2(b[e] - v[e]);
         }
     }
-    reduce_finish(s, fin, ra);
+    reduce_partial(s, ra);
 }
 
 template <typename R>
@@ -133,7 +134,21 @@ __global__ __launch_bounds__(kBlock) void k_copy(const cx<R> *__restrict__ src,
     for (int c = 0; c < 12; ++c) dst[c * S + k] = src[c * S + k];
 }
 
+// Sum the n partials in workgroup order and apply `fin` (skipped once the CG has finished).
+__global__ __launch_bounds__(kBlock) void k_finish(RedArgs ra, int n, int fin, const int *done) {
+    if (done && *done) return;
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += ra.partials[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) fin_apply(fin, s, ra);
+}
+
 // ------------------------------------------------------------------- launchers ---------
+
+hipError_t launch_finish(const RedArgs &ra, int nparts, int fin, const int *done, hipStream_t s) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, s, ra, nparts, fin, done);
+    return hipGetLastError();
+}
 
 static inline dim3 grid_sites(const Geom &g) { return dim3(nblocks(g)); }
 
@@ -193,7 +208,8 @@ hipError_t launch_norm2(int prec, const Geom &g, const void *f, const RedArgs &r
                            (const cx<float> *)f, g.vh, g.stride, ra, fin, done);
     else
         return hipErrorInvalidValue;
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finish(ra, nblocks(g), fin, done, s);
 }
 
 hipError_t launch_cg_init(const Geom &g, const void *b, const void *DDp, void *r, void *p,
@@ -201,7 +217,8 @@ hipError_t launch_cg_init(const Geom &g, const void *b, const void *DDp, void *r
     hipLaunchKernelGGL(k_cg_init, grid_sites(g), dim3(kBlock), 0, s, (const cx<double> *)b,
                        (const cx<double> *)DDp, (cx<double> *)r, (cx<double> *)p, g.vh,
                        g.stride, ra);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finish(ra, nblocks(g), FIN_RR, nullptr, s);
 }
 
 hipError_t launch_cg_xpay(const Geom &g, void *p, const void *r, const CgState *st,
@@ -216,7 +233,8 @@ hipError_t launch_diff_norm2(const Geom &g, const void *b, const void *v, const 
                              int fin, hipStream_t s) {
     hipLaunchKernelGGL(k_diff_norm2, grid_sites(g), dim3(kBlock), 0, s, (const cx<double> *)b,
                        (const cx<double> *)v, g.vh, g.stride, ra, fin);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finish(ra, nblocks(g), fin, nullptr, s);
 }
 
 hipError_t launch_copy(int prec, const Geom &g, const void *src, void *dst, hipStream_t s) {

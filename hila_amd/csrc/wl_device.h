@@ -183,7 +183,16 @@ struct Geom {
     int vh;          // sites per parity
     int stride;      // elements between two component planes of a field (>= vh, padded)
     int antiperiodic;
+    // Traversal of the hopping kernel: a wave owns one 64-site segment of the checkerboard
+    // (64 consecutive k: coalesced plane accesses).  qc = 0 walks segments in memory order;
+    // qc > 0 walks them chunk by chunk, each chunk = qc consecutive segments of a time slice,
+    // taken for every t before the next chunk, so that the t neighbours of a site are read
+    // while their spinors are still in the XCD's L2.  sps = segments per time slice,
+    // ns = segments per parity (both valid only when qc > 0).
+    int qc, sps, ns;
+    int blk;         // workgroup size of the hopping kernel (64, 128 or 256)
 };
+
 
 struct SiteNb {
     int fwd[4], bwd[4];   // checkerboard index of x+mu / x-mu (opposite parity)
@@ -225,6 +234,18 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// Checkerboard site index handled by this lane of a hopping-kernel launch (>= vh: idle lane).
+__device__ __forceinline__ int lane_site(const Geom &g) {
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (g.qc <= 0) return tile * blockDim.x + threadIdx.x;
+    const int o = tile * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    if (o >= g.ns) return g.vh;
+    const int per = g.qc * g.L[3];
+    const int c = o / per, rem = o - c * per;
+    const int t = rem / g.qc, q = rem - t * g.qc;
+    return ((t * g.sps + c * g.qc + q) << 6) + (int)(threadIdx.x & 63);
+}
+
 // ------------------------------------------------------------------- reductions ---------
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -250,10 +271,6 @@ __device__ __forceinline__ double block_sum(double v) {
     return s;
 }
 
-__device__ __forceinline__ double load_coherent(const double *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Device-resident scalars of one CG solve (conjugate_gradient.h:55-57), updated by the last
 // workgroup of each reduction kernel so that no scalar leaves the device inside the loop.
 struct CgState {
@@ -274,8 +291,7 @@ enum Fin : int {
 };
 
 struct RedArgs {
-    double *partials;   // one slot per workgroup
-    unsigned *ticket;   // zero between reductions
+    double *partials;   // one slot per workgroup of the reducing kernel
     CgState *st;
     double *out;        // FIN_STORE target
 };
@@ -309,29 +325,14 @@ __device__ __forceinline__ void fin_apply(int fin, double s, const RedArgs &ra) 
     }
 }
 
-// Publish this workgroup's partial sum; the last workgroup to arrive sums all partials in
-// workgroup order (deterministic) and applies `fin`.  Every thread must call it.
-__device__ __forceinline__ void reduce_finish(double v, int fin, const RedArgs &ra) {
-    __shared__ int am_last;
+// Each workgroup of a reducing kernel stores its partial sum in partials[blockIdx.x]; a
+// one-workgroup finish kernel then adds them in workgroup order and applies the CG scalar
+// update (deterministic; no atomics).  A last-workgroup-finishes scheme would need a
+// device-scope release fence in every workgroup, which on this GPU writes back the XCD's L2
+// and stalls a memory-bound kernel.  Every thread must call it.
+__device__ __forceinline__ void reduce_partial(double v, const RedArgs &ra) {
     const double bs = block_sum(v);
-    const int nblk = gridDim.x;
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(ra.partials + blockIdx.x, bs, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned t = atomicAdd(ra.ticket, 1u);
-        am_last = (t == (unsigned)nblk - 1u);
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __threadfence();
-    double s = 0;
-    for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += load_coherent(ra.partials + i);
-    s = block_sum(s);
-    if (threadIdx.x == 0) {
-        fin_apply(fin, s, ra);
-        __hip_atomic_store(ra.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (threadIdx.x == 0) ra.partials[blockIdx.x] = bs;
 }
 
 }  // namespace wl

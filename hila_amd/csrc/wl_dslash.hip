@@ -107,7 +107,7 @@ __device__ __forceinline__ void hop_chain(cx<R> (&acc)[4][3], HopBuf<R> (&buf)[2
 template <typename R, int SIGN, int EPI>
 __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
     if (A.done && *A.done) return;
-    const int k = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int k = lane_site(A.g);
     const int S = A.g.stride;
     double nrm = 0;
     if (k < A.g.vh) {
@@ -156,15 +156,15 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
                 }
         }
     }
-    if constexpr (EPI == EPI_AXPY_NORM || EPI == EPI_CGUPD) reduce_finish(nrm, A.fin, A.red);
+    if constexpr (EPI == EPI_AXPY_NORM || EPI == EPI_CGUPD) reduce_partial(nrm, A.red);
 }
 
-// Workgroup size: 256 lanes when that still gives >= 4 workgroups per CU on the 256 CUs,
-// otherwise smaller workgroups so that small lattices still spread over all CUs.
-int dslash_block(const Geom &g) {
-    if (g.vh >= 256 * 1024) return 256;
-    if (g.vh >= 128 * 256) return 128;
-    return 64;
+// Default workgroup size, from measurements on MI355X (256 CUs): 256 lanes from 2^19 sites
+// per parity up, 64 lanes for mid-size lattices, 128 below (where a launch is latency-bound).
+int dslash_block_auto(int vh) {
+    if (vh >= 512 * 1024) return 256;
+    if (vh >= 128 * 1024) return 64;
+    return 128;
 }
 
 int nblocks(const Geom &g) { return (g.vh + kBlock - 1) / kBlock; }
@@ -185,10 +185,12 @@ static hipError_t launch_t(const Geom &g, const DslashLaunch &L, hipStream_t s) 
     A.r = (cx<double> *)L.r;
     A.red = L.red;
     A.fin = L.fin;
-    const int blk = dslash_block(g);
+    const int blk = g.blk;
     const int nb = (g.vh + blk - 1) / blk;
     hipLaunchKernelGGL((k_dslash<R, SIGN, EPI>), dim3(nb), dim3(blk), 0, s, A);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || (EPI != EPI_AXPY_NORM && EPI != EPI_CGUPD)) return e;
+    return launch_finish(L.red, nb, L.fin, L.done, s);
 }
 
 template <typename R, int SIGN>

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <mutex>
 #include <string>
+#include <unordered_set>
 
 #include "wl_device.h"
 #include "wilson_mi355x.h"
@@ -74,14 +75,18 @@ hipError_t launch_diff_norm2(const Geom &g, const void *b, const void *v, const 
 // copy f64 even field v into the all-zero-if-null convention: y = v
 hipError_t launch_copy(int prec, const Geom &g, const void *src, void *dst, hipStream_t s);
 
+// one-workgroup kernel: sum nparts partials in order, apply `fin` (skipped when *done != 0)
+hipError_t launch_finish(const RedArgs &ra, int nparts, int fin, const int *done, hipStream_t s);
+
 int nblocks(const Geom &g);
+int dslash_block_auto(int vh);
 
 }  // namespace wl
 
 // ---------------------------------------------------------------- ABI objects ---------
 
 struct wl_field {
-    wl_ctx *ctx;
+    wl_ctx *ctx;     // null once the context has been destroyed (orphaned handle)
     int parity;  // WL_EVEN / WL_ODD
     int prec;
     int nrhs;
@@ -106,7 +111,6 @@ struct wl_ctx {
     void *staging = nullptr;
     size_t staging_bytes = 0;
     double *partials = nullptr;
-    unsigned *ticket = nullptr;
     wl::CgState *st = nullptr;
     wl::CgState *st_host = nullptr;  // pinned
     double *scratch_d = nullptr;     // FIN_STORE target
@@ -115,5 +119,7 @@ struct wl_ctx {
     wl_field *tmp_odd64 = nullptr;   // odd temporary of M
     wl_field *tmp_odd32 = nullptr;
     wl_field *cg_r = nullptr, *cg_p = nullptr, *cg_Dp = nullptr, *cg_DDp = nullptr;
+    // fields created through wl_field_create and not yet destroyed; wl_destroy orphans them
+    std::unordered_set<wl_field *> user_fields;
     std::mutex mu;
 };

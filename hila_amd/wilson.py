@@ -27,12 +27,13 @@ LIB_PATH = os.path.join(_HERE, "lib", "libhila_wilson.so")
 WL_OK, WL_E_ARG, WL_E_OOM, WL_E_DEVICE, WL_E_NOCONV, WL_E_STATE, WL_E_NOTIMPL = 0, -1, -2, -3, -4, -5, -6
 EVEN, ODD = 1, 2
 F64, F32, F16 = 0, 1, 2
+OPT_TRAVERSAL_CHUNK, OPT_BLOCK = 1, 2
 
 EXPORTED_SYMBOLS = [
     "wl_version", "wl_last_error", "wl_create", "wl_destroy", "wl_sync", "wl_volume",
     "wl_gauge_upload", "wl_field_create", "wl_field_destroy", "wl_field_upload",
     "wl_field_download", "wl_field_zero", "wl_field_copy", "wl_field_norm2", "wl_dslash_eo", "wl_meo_apply",
-    "wl_cg_solve", "wl_time_op", "wl_gen_gauge", "wl_gen_spinor", "wl_gen_point",
+    "wl_cg_solve", "wl_set_option", "wl_get_option", "wl_time_op", "wl_gen_gauge", "wl_gen_spinor", "wl_gen_point",
 ]
 
 
@@ -81,6 +82,8 @@ def lib():
         L.wl_dslash_eo.argtypes = [vp, vp, vp, d, i]
         L.wl_meo_apply.argtypes = [vp, vp, vp, d, i]
         L.wl_cg_solve.argtypes = [vp, vp, vp, d, d, i, C.POINTER(CGStats)]
+        L.wl_set_option.argtypes = [vp, i, i]
+        L.wl_get_option.argtypes = [vp, i, C.POINTER(i)]
         L.wl_time_op.argtypes = [vp, i, vp, vp, d, i, i, i, C.POINTER(d)]
         L.wl_gen_gauge.argtypes = [ip4, i, d, C.c_uint64, C.POINTER(vp)]
         L.wl_gen_spinor.argtypes = [ip4, C.c_uint64, i, vp]
@@ -169,6 +172,14 @@ class Lattice:
 
     def sync(self):
         check(lib().wl_sync(self.handle))
+
+    def set_option(self, option, value):
+        check(lib().wl_set_option(self.handle, int(option), int(value)))
+
+    def get_option(self, option):
+        v = C.c_int(0)
+        check(lib().wl_get_option(self.handle, int(option), C.byref(v)))
+        return v.value
 
     def close(self):
         if self._h is not None:

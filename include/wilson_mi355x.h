@@ -106,6 +106,9 @@ int wl_gauge_upload(wl_ctx *ctx, const double *const U[4]);
 /* A device Wilson-vector field on the sites of one parity (WL_EVEN / WL_ODD), holding nrhs
  * independent vectors, stored in precision prec. */
 int wl_field_create(wl_ctx *ctx, int parity, int prec, int nrhs, wl_field **out);
+/* Fields should be destroyed before their context.  A field still alive when wl_destroy runs
+ * loses its device memory and becomes an orphan: every call on it returns WL_E_STATE except
+ * wl_field_destroy, which releases the handle. */
 int wl_field_destroy(wl_field *f);
 /* Copy rhs vector `rhs` of f from / to host AoS buffer `host` of V/2 sites x 24 doubles, i.e.
  * the parity block of a HILA Field<WilsonVector<3,double>> buffer: its first V/2 sites for
@@ -140,6 +143,19 @@ int wl_meo_apply(wl_ctx *ctx, wl_field *out, const wl_field *in, double kappa, i
  * and stats filled) if maxiters is reached first for any rhs. */
 int wl_cg_solve(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa, double accuracy,
                 int maxiters, wl_cg_stats *stats);
+
+/* ---- kernel options ------------------------------------------------------------------- */
+
+/* Tuning knobs of the hopping kernel (no counterpart in HILA, whose GPU loops are generated
+ * with fixed N_threads = 256, libraries/plumbing/params.h:248):
+ *   WL_OPT_TRAVERSAL_CHUNK  order in which the 64-site segments of the checkerboard are
+ *       visited: 0 = memory order; n > 0 = chunks of n segments of one time slice, visited for
+ *       every t before the next chunk (n must divide the segments per slice); -1 = default.
+ *   WL_OPT_BLOCK  workgroup size 64, 128 or 256; 0 = default for the volume.
+ * Results do not depend on either option; only the speed does. */
+enum { WL_OPT_TRAVERSAL_CHUNK = 1, WL_OPT_BLOCK = 2 };
+int wl_set_option(wl_ctx *ctx, int option, int value);
+int wl_get_option(const wl_ctx *ctx, int option, int *value);
 
 /* ---- diagnostics ---------------------------------------------------------------------- */
 

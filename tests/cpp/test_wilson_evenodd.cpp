@@ -1,0 +1,116 @@
+// The Wilson even-odd checks of HILA's applications/test_cases/test_CG.cpp:215-288, written
+// against the C++ mirror (hila_wilson.hpp) of Dirac_Wilson_evenodd and CG: D / D^dag
+// adjointness, then D (DdgD)^-1 Ddg = 1, DdgD (DdgD)^-1 = 1 and (DdgD)^-1 DdgD = 1, for unit
+// links (as test_CG.cpp:62-65) and for Haar-random links.  Prints PASS and returns 0 when all
+// hold.  The 1e-16 bounds of test_CG.cpp on squared differences are applied relative to the
+// squared norm of the vector compared.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <vector>
+
+#include "hila_wilson.hpp"
+
+using namespace hila_wilson;
+
+static int failures = 0;
+#define CHECK(cond, what)                                                            \
+    do {                                                                             \
+        if (!(cond)) {                                                               \
+            std::printf("FAIL: %s (%s:%d)\n", what, __FILE__, __LINE__);             \
+            ++failures;                                                              \
+        } else {                                                                     \
+            std::printf("ok:   %s\n", what);                                         \
+        }                                                                            \
+    } while (0)
+
+// sum over even sites of a[X].dot(b[X]) (matrix.h:1395-1403: conj(a) . b)
+static std::complex<double> dot_even(const WilsonField &a, const WilsonField &b) {
+    const std::vector<double> ha = a.host(), hb = b.host();
+    const size_t n = ha.size() / 2;  // even sites come first
+    std::complex<double> s = 0;
+    for (size_t i = 0; i < n; i += 2)
+        s += std::conj(std::complex<double>(ha[i], ha[i + 1])) * std::complex<double>(hb[i], hb[i + 1]);
+    return s;
+}
+
+static double diff2_even(const WilsonField &a, const WilsonField &b) {
+    const std::vector<double> ha = a.host(), hb = b.host();
+    double s = 0;
+    for (size_t i = 0; i < ha.size() / 2; ++i) s += (ha[i] - hb[i]) * (ha[i] - hb[i]);
+    return s;
+}
+
+static void run(Lattice &lattice, int link_kind, const char *name) {
+    std::printf("Checking with Dirac_Wilson_evenodd, %s links\n", name);
+    GaugeField U(lattice);
+    U.random(link_kind, 0.0, 7);
+    using dirac = Dirac_Wilson_evenodd<SU3d>;
+    dirac D(0.12, U);
+    WilsonField a(lattice), b(lattice), Db(lattice), Ddaggera(lattice), DdaggerDb(lattice);
+
+    a.zero(ODD);
+    b.zero(ODD);
+    a.gaussian_random(11, EVEN);
+    b.gaussian_random(12, EVEN);
+
+    D.apply(b, Db);
+    D.dagger(a, Ddaggera);
+    const std::complex<double> d = dot_even(a, Db) - dot_even(Ddaggera, b);
+    const double scale = std::abs(dot_even(a, Db));
+    CHECK(std::norm(d) < 1e-26 * scale * scale, "test Dirac_Wilson_evenodd (adjoint)");
+
+    // Now check that D (DdgD)^-1 Ddg = 1
+    CG<dirac> inverse(D);
+    a.zero(ALL);
+    b.zero(ALL);
+    a.gaussian_random(13, EVEN);
+    D.dagger(a, Ddaggera);
+    inverse.apply(Ddaggera, b);
+    D.apply(b, Db);
+    double diffre = diff2_even(a, Db);
+    std::printf("D inv Dg %g (%d iterations)\n", diffre, inverse.stats.iterations);
+    CHECK(diffre < 1e-16 * a.squarenorm(EVEN), "test D (DdgD)^-1 Ddg");
+
+    // Run CG on a, multiply by DdaggerD and check the same
+    a.gaussian_random(14, EVEN);
+    b.zero(ALL);
+    inverse.apply(a, b);
+    D.apply(b, Db);
+    D.dagger(Db, DdaggerDb);
+    diffre = diff2_even(a, DdaggerDb);
+    CHECK(diffre < 1e-16 * a.squarenorm(EVEN), "test DdgD (DdgD)^-1");
+
+    // The other way around, multiply first
+    b.gaussian_random(15, EVEN);
+    D.apply(b, Db);
+    D.dagger(Db, DdaggerDb);
+    a.zero(ALL);
+    inverse.apply(DdaggerDb, a);
+    diffre = diff2_even(a, b);
+    CHECK(diffre < 1e-16 * b.squarenorm(EVEN), "test (DdgD)^-1 DdgD");
+    CHECK(inverse.stats.converged == 1 && inverse.stats.true_resid < 1e-11,
+          "CG true residual below accuracy");
+
+    // apply leaves the odd half of the output at zero (wilson.h:278)
+    Db.gaussian_random(16, ALL);
+    D.apply(b, Db);
+    CHECK(Db.squarenorm(ODD) == 0.0, "out[ODD] = 0");
+}
+
+int main() {
+    try {
+        Lattice lattice({16, 8, 8, 8});  // test_CG.cpp:57, NDIM == 4
+        run(lattice, 0, "unit");
+        run(lattice, 1, "Haar random");
+    } catch (const error &e) {
+        std::printf("FAIL: exception %s\n", e.what());
+        return 2;
+    }
+    if (failures) {
+        std::printf("%d FAILED\n", failures);
+        return 1;
+    }
+    std::printf("PASS\n");
+    return 0;
+}

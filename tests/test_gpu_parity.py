@@ -198,3 +198,42 @@ def test_argument_errors():
         D.apply(o, e)
     with pytest.raises(W.WilsonError):
         W.Lattice((4, 4, 4, 5))
+
+
+@pytest.mark.parametrize("dims", [(32, 32, 32, 32), (24, 24, 24, 48), (16, 16, 16, 16)],
+                         ids=lambda d: "x".join(map(str, d)))
+def test_traversal_and_workgroup_options_do_not_change_results(dims):
+    """The hopping kernel's site traversal (memory order or (chunk, t)-tiled) and workgroup
+    size change only the speed: the output is bitwise identical."""
+    lat, U, _ = _setup(dims, "smooth", 0.3, 3, True)
+    del U
+    D = W.Dirac_Wilson_evenodd(0.125, lat)
+    psi = W.gen_spinor(dims, 4, EVEN)
+    fin, fo, fe = W.Field(lat).upload(psi), W.Field(lat, ODD), W.Field(lat)
+    sps = dims[0] * dims[1] * dims[2] // 2 // 64
+    ref_o = ref_e = None
+    for blk in (64, 128, 256):
+        for qc in sorted({0, -1} | {q for q in (4, 27, 32, 128) if sps % q == 0}):
+            lat.set_option(W.OPT_TRAVERSAL_CHUNK, qc)
+            lat.set_option(W.OPT_BLOCK, blk)
+            D.hop_set(fin, fo, 1)
+            D.apply(fin, fe)
+            o, e = fo.download(), fe.download()
+            if ref_o is None:
+                ref_o, ref_e = o, e
+            assert np.array_equal(o, ref_o) and np.array_equal(e, ref_e), (blk, qc)
+    with pytest.raises(W.WilsonError):
+        lat.set_option(W.OPT_BLOCK, 96)
+
+
+def test_field_outliving_its_context_is_orphaned():
+    """wl_destroy releases the device memory of fields still alive and orphans the handles:
+    calls on them return WL_E_STATE, and destroying them afterwards is safe."""
+    lat = W.Lattice((4, 4, 4, 4))
+    f = W.Field(lat)
+    f.upload(W.gen_spinor((4, 4, 4, 4), 1, EVEN))
+    lat.close()
+    with pytest.raises(W.WilsonError) as e:
+        f.download()
+    assert e.value.code == W.WL_E_STATE
+    f.close()
