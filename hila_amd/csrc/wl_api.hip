@@ -241,7 +241,8 @@ extern "C" int wl_destroy(wl_ctx *c) {
     if (!c) return WL_OK;
     DeviceGuard dg(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (wl_field *f : {c->tmp_odd64, c->tmp_odd32, c->cg_r, c->cg_p, c->cg_Dp, c->cg_DDp})
+    for (wl_field *f : {c->tmp_odd64, c->tmp_odd32, c->cg_r, c->cg_p, c->cg_Dp, c->cg_DDp,
+                        c->mx_r, c->mx_p, c->mx_Dp})
         field_free(f);
     // fields the caller still holds: release their device memory and orphan the handles
     for (wl_field *f : c->user_fields) {
@@ -500,7 +501,7 @@ int cg_f64(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int
             L.fin = FIN_RRNEW;
             HIPCHK(launch_dslash(WL_F64, c->g, L, c->stream));
             // p = beta p + r
-            HIPCHK(launch_cg_xpay(c->g, p, r, c->st, done, c->stream));
+            HIPCHK(launch_cg_xpay(WL_F64, c->g, p, r, c->st, c->stream));
         }
         enqueued += n;
         HIPCHK(hipMemcpyAsync(&c->st_host->done, &c->st->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -528,6 +529,126 @@ int cg_f64(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int
     }
     if (!conv) return fail(WL_E_NOCONV, "CG did not converge in %d iterations (rr_rel %.3e)", S.iter,
                            S.rrnew / S.source_norm);
+    return WL_OK;
+}
+
+// Mixed-precision CG on M^dag M with reliable updates (FP64 solution, FP64 reductions, inner
+// operator and Krylov vectors r, p, Dp in `prec`).  The recurrences are those of HILA's CG
+// (conjugate_gradient.h:75-95) run in the inner precision; the finish kernel of each iteration
+// asks for a reliable update when ||r|| has dropped by `delta` since the last one, after
+// `update_every` iterations, or when the iterated residual meets the target.  An update
+// recomputes r = b - M^dag M x with the FP64 operator, restarts rr and beta from it and
+// declares convergence only on that true residual.
+int cg_mixed(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int maxiters,
+             int prec, double delta, int update_every, wl_cg_stats *stats) {
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    if ((rc = ensure_gauge(c, prec))) return rc;
+    for (wl_field **w : {&c->cg_Dp, &c->cg_DDp})
+        if ((rc = ensure_tmp(c, WL_F64, w, WL_EVEN))) return rc;
+    if ((rc = ensure_tmp(c, WL_F64, &c->tmp_odd64, WL_ODD))) return rc;
+    for (wl_field **w : {&c->mx_r, &c->mx_p, &c->mx_Dp}) {
+        if (*w && (*w)->prec != prec) {
+            field_free(*w);
+            *w = nullptr;
+        }
+        if ((rc = ensure_tmp(c, prec, w, WL_EVEN))) return rc;
+    }
+    // odd scratch of the inner operator (shared with the FP64 update when prec is FP64: the
+    // stream orders their uses)
+    wl_field **todd = prec == WL_F64 ? &c->tmp_odd64 : &c->tmp_odd32;
+    if ((rc = ensure_tmp(c, prec, todd, WL_ODD))) return rc;
+    void *r = c->mx_r->data, *p = c->mx_p->data, *Dp = c->mx_Dp->data, *t = (*todd)->data;
+    void *Dp64 = c->cg_Dp->data, *DDp64 = c->cg_DDp->data, *t64 = c->tmp_odd64->data;
+    const RedArgs ra = red_args(c);
+    const int *done = &c->st->done;
+
+    CgState init{};
+    init.acc2 = accuracy * accuracy;
+    init.maxiters = maxiters;
+    init.delta2 = delta * delta;
+    init.update_every = update_every;
+    *c->st_host = init;
+    HIPCHK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_norm2(WL_F64, c->g, b, ra, FIN_SOURCE, nullptr, c->stream));
+
+    // reliable update: r = b - M^dag M x in FP64, r_lo = r, restart (or converged)
+    auto reliable_update = [&]() -> int {
+        int e;
+        if ((e = meo_one(c, WL_F64, Dp64, x, t64, kappa, +1, EPI_AXPY, nullptr, FIN_NONE))) return e;
+        if ((e = meo_one(c, WL_F64, DDp64, Dp64, t64, kappa, -1, EPI_AXPY, nullptr, FIN_NONE))) return e;
+        HIPCHK(launch_residual_update(prec, c->g, b, DDp64, r, ra, c->stream));
+        return WL_OK;
+    };
+    if ((rc = reliable_update())) return rc;
+    HIPCHK(launch_copy(prec, c->g, r, p, c->stream));  // p = r
+    HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->st_host->source_norm == 0.0) {
+        HIPCHK(hipMemsetAsync(x, 0, c->cg_Dp->bytes, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->converged = 1;
+        }
+        return WL_OK;
+    }
+    int done_h = c->st_host->done;
+    if (maxiters <= 0 && done_h == 0) done_h = 2;
+
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    while (done_h == 0 || done_h == 3 || done_h == 4) {
+        if (done_h == 3 || done_h == 4) {
+            if ((rc = reliable_update())) return rc;
+            HIPCHK(launch_cg_xpay(prec, c->g, p, r, c->st, c->stream));  // skipped if converged
+        }
+        for (int it = 0; it < kChunk; ++it) {
+            if ((rc = meo_one(c, prec, Dp, p, t, kappa, +1, EPI_AXPY_NORM, done, FIN_ALPHA))) return rc;
+            if ((rc = dslash_one(c, prec, t, 1, Dp, nullptr, -kappa, -1, EPI_SET, done, FIN_NONE)))
+                return rc;
+            DslashLaunch L{};
+            L.in = t;
+            L.xin = Dp;
+            L.gauge = gauge_for(c, prec);
+            L.a = kappa;
+            L.out_par = 0;
+            L.sign = -1;
+            L.epi = EPI_CGUPD;
+            L.done = done;
+            L.x = x;
+            L.p = p;
+            L.r = r;
+            L.red = ra;
+            L.fin = FIN_RRNEW_MIXED;
+            HIPCHK(launch_dslash(prec, c->g, L, c->stream));
+            HIPCHK(launch_cg_xpay(prec, c->g, p, r, c->st, c->stream));
+        }
+        HIPCHK(hipMemcpyAsync(&c->st_host->done, &c->st->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipEventRecord(ev, c->stream));
+        HIPCHK(hipEventSynchronize(ev));
+        done_h = c->st_host->done;
+    }
+    (void)hipEventDestroy(ev);
+    if (done_h == 2) {  // maxiters: report the true residual of the final x
+        if ((rc = reliable_update())) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const CgState &S = *c->st_host;
+    const bool conv = S.done == 1;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->iterations = S.iter;
+        stats->converged = conv ? 1 : 0;
+        stats->rr_rel = S.rrnew / S.source_norm;
+        stats->true_resid = std::sqrt(S.true_rr / S.source_norm);
+        stats->reliable_updates = S.nreliable - 1;  // the initial residual is not an update
+        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (!conv)
+        return fail(WL_E_NOCONV, "mixed CG did not converge in %d iterations (true residual %.3e)",
+                    S.iter, std::sqrt(S.true_rr / S.source_norm));
     return WL_OK;
 }
 
@@ -627,4 +748,33 @@ extern "C" int wl_get_option(const wl_ctx *c, int option, int *value) {
     case WL_OPT_BLOCK: *value = c->g.blk; return WL_OK;
     default: return fail(WL_E_ARG, "unknown option %d", option);
     }
+}
+
+extern "C" int wl_cg_solve_mixed(wl_ctx *c, wl_field *x, const wl_field *b, double kappa,
+                                 double accuracy, int maxiters, int inner_prec, double delta,
+                                 int update_every, wl_cg_stats *stats) {
+    if (!c || !x || !b) return fail(WL_E_ARG, "null argument");
+    if (x->ctx != c || b->ctx != c) return fail(WL_E_ARG, "field belongs to another context");
+    if (x->parity != WL_EVEN || b->parity != WL_EVEN) return fail(WL_E_ARG, "CG acts on even fields");
+    if (x == b) return fail(WL_E_ARG, "x and b must be different fields");
+    if (x->prec != WL_F64 || b->prec != WL_F64) return fail(WL_E_ARG, "x and b must be FP64 fields");
+    if (x->nrhs != b->nrhs) return fail(WL_E_ARG, "x and b differ in nrhs");
+    if (inner_prec != WL_F32 && inner_prec != WL_F64)
+        return fail(WL_E_NOTIMPL, "inner precision %d", inner_prec);
+    if (!(accuracy > 0) || !(delta > 0 && delta < 1) || update_every < 1)
+        return fail(WL_E_ARG, "need accuracy > 0, 0 < delta < 1, update_every >= 1");
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    int rc = ensure_gauge(c, WL_F64);
+    if (rc) return rc;
+    int worst = WL_OK;
+    for (int r = 0; r < b->nrhs; ++r) {
+        wl_cg_stats s{};
+        rc = cg_mixed(c, x->rhs_ptr(r), b->rhs_ptr(r), kappa, accuracy, maxiters, inner_prec, delta,
+                      update_every, &s);
+        if (rc && rc != WL_E_NOCONV) return rc;
+        if (rc) worst = rc;
+        if (stats) stats[r] = s;
+    }
+    return worst;
 }

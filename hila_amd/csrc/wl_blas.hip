@@ -96,18 +96,40 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(const cx<double> *__restrict
     reduce_partial(s, ra);
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_xpay(cx<double> *__restrict__ p,
-                                                    const cx<double> *__restrict__ r,
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_cg_xpay(cx<R> *__restrict__ p,
+                                                    const cx<R> *__restrict__ r,
                                                     const CgState *st, int vh, int S) {
     if (st->done) return;
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= vh) return;
-    const double beta = st->beta;
+    const R beta = (R)st->beta;
 #pragma unroll
     for (int c = 0; c < 12; ++c) {
         const int e = c * S + k;
         p[e] = beta * p[e] + r[e];
     }
+}
+
+// Reliable update: r = b - DDp in FP64 (DDp = M^dag M x in FP64), stored in the inner
+// precision R, ||r||^2 reduced in FP64.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_residual_update(const cx<double> *__restrict__ b,
+                                                            const cx<double> *__restrict__ ddp,
+                                                            cx<R> *__restrict__ r, int vh, int S,
+                                                            RedArgs ra) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    double s = 0;
+    if (k < vh) {
+#pragma unroll
+        for (int c = 0; c < 12; ++c) {
+            const int e = c * S + k;
+            const cx<double> v = b[e] - ddp[e];
+            r[e] = {(R)v.x, (R)v.y};
+            s += norm2(v);
+        }
+    }
+    reduce_partial(s, ra);
 }
 
 __global__ __launch_bounds__(kBlock) void k_diff_norm2(const cx<double> *__restrict__ b,
@@ -221,12 +243,33 @@ hipError_t launch_cg_init(const Geom &g, const void *b, const void *DDp, void *r
     return e != hipSuccess ? e : launch_finish(ra, nblocks(g), FIN_RR, nullptr, s);
 }
 
-hipError_t launch_cg_xpay(const Geom &g, void *p, const void *r, const CgState *st,
-                          const int *done, hipStream_t s) {
-    (void)done;
-    hipLaunchKernelGGL(k_cg_xpay, grid_sites(g), dim3(kBlock), 0, s, (cx<double> *)p,
-                       (const cx<double> *)r, st, g.vh, g.stride);
+hipError_t launch_cg_xpay(int prec, const Geom &g, void *p, const void *r, const CgState *st,
+                          hipStream_t s) {
+    if (prec == WL_F64)
+        hipLaunchKernelGGL(k_cg_xpay<double>, grid_sites(g), dim3(kBlock), 0, s, (cx<double> *)p,
+                           (const cx<double> *)r, st, g.vh, g.stride);
+    else if (prec == WL_F32)
+        hipLaunchKernelGGL(k_cg_xpay<float>, grid_sites(g), dim3(kBlock), 0, s, (cx<float> *)p,
+                           (const cx<float> *)r, st, g.vh, g.stride);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+hipError_t launch_residual_update(int prec, const Geom &g, const void *b, const void *DDp,
+                                  void *r_lo, const RedArgs &ra, hipStream_t s) {
+    if (prec == WL_F64)
+        hipLaunchKernelGGL(k_residual_update<double>, grid_sites(g), dim3(kBlock), 0, s,
+                           (const cx<double> *)b, (const cx<double> *)DDp, (cx<double> *)r_lo,
+                           g.vh, g.stride, ra);
+    else if (prec == WL_F32)
+        hipLaunchKernelGGL(k_residual_update<float>, grid_sites(g), dim3(kBlock), 0, s,
+                           (const cx<double> *)b, (const cx<double> *)DDp, (cx<float> *)r_lo,
+                           g.vh, g.stride, ra);
+    else
+        return hipErrorInvalidValue;
+    hipError_t e = hipGetLastError();
+    return e != hipSuccess ? e : launch_finish(ra, nblocks(g), FIN_UPDATE, nullptr, s);
 }
 
 hipError_t launch_diff_norm2(const Geom &g, const void *b, const void *v, const RedArgs &ra,

@@ -273,10 +273,17 @@ __device__ __forceinline__ double block_sum(double v) {
 
 // Device-resident scalars of one CG solve (conjugate_gradient.h:55-57), updated by the last
 // workgroup of each reduction kernel so that no scalar leaves the device inside the loop.
+// `done` stops every gated kernel of the loop while non-zero:
+//   1 converged, 2 maxiters reached, 3 reliable update requested (mixed precision),
+//   4 iterated residual below the target, to be confirmed by a reliable update.
 struct CgState {
     double source_norm, target_rr, rr, pDp, rrnew, alpha, beta, acc2;
     double true_rr, scratch;
+    double rr_update;   // ||r||^2 at the last reliable update
+    double delta2;      // update when ||r||^2 < delta2 * rr_update
     int iter, done, maxiters, nreliable;
+    int iter_since;     // iterations since the last reliable update
+    int update_every;   // ... or after this many iterations
 };
 
 // What the last workgroup of a reduction does with the reduced sum.
@@ -288,6 +295,8 @@ enum Fin : int {
     FIN_ALPHA = 4,     // pDp = s; alpha = rr / pDp
     FIN_RRNEW = 5,     // rrnew = s; convergence test and beta (conjugate_gradient.h:87-95)
     FIN_TRUE = 6,      // true_rr = s
+    FIN_RRNEW_MIXED = 7,  // rrnew = s; convergence / reliable-update request / beta
+    FIN_UPDATE = 8,    // s = ||b - M^dag M x||^2 after a reliable update: converged or restart
 };
 
 struct RedArgs {
@@ -321,6 +330,36 @@ __device__ __forceinline__ void fin_apply(int fin, double s, const RedArgs &ra) 
         }
         break;
     case FIN_TRUE: st->true_rr = s; break;
+    case FIN_RRNEW_MIXED:
+        st->rrnew = s;
+        st->iter += 1;
+        st->iter_since += 1;
+        if (s < st->target_rr) {
+            st->done = 4;
+        } else if (s < st->delta2 * st->rr_update || st->iter_since >= st->update_every) {
+            st->done = 3;
+        } else {
+            st->beta = s / st->rr;
+            st->rr = s;
+            if (st->iter >= st->maxiters) st->done = 2;
+        }
+        break;
+    case FIN_UPDATE:
+        st->true_rr = s;
+        st->nreliable += 1;
+        if (s < st->target_rr) {
+            st->rrnew = s;
+            st->done = 1;
+        } else {
+            // restart the recurrences from the true residual: beta = r2_true / r2_previous
+            st->beta = s / st->rr;
+            st->rr = s;
+            st->rrnew = s;
+            st->rr_update = s;
+            st->iter_since = 0;
+            st->done = st->iter >= st->maxiters ? 2 : 0;
+        }
+        break;
     default: break;
     }
 }

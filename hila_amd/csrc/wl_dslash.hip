@@ -33,9 +33,9 @@ struct DslashArgs {
     Geom g;
     int out_par;
     const int *done;
-    cx<double> *__restrict__ x;
-    const cx<double> *__restrict__ p;
-    cx<double> *__restrict__ r;
+    cx<double> *__restrict__ x;   // the solution is accumulated in FP64 at every precision
+    const cx<R> *__restrict__ p;
+    cx<R> *__restrict__ r;
     RedArgs red;
     int fin;
 };
@@ -141,18 +141,21 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
                     stc<true>(A.out + e, o);
                     if constexpr (EPI == EPI_AXPY_NORM) nrm += (double)norm2(o);
                 }
-        } else {  // EPI_CGUPD, FP64
+        } else {  // EPI_CGUPD: x (FP64) += alpha p, r -= alpha DDp in R
             const double alpha = A.red.st->alpha;
+            const R alpha_r = (R)alpha;
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const int e = (s * 3 + c) * S + k;
-                    const cx<double> ddp = ldc<true>(A.xin + e) + a * acc[s][c];
-                    stc<true>(A.x + e, ldc<true>(A.x + e) + alpha * ldc<true>(A.p + e));
-                    const cx<double> rn = ldc<true>(A.r + e) - alpha * ddp;
+                    const cx<R> ddp = ldc<true>(A.xin + e) + a * acc[s][c];
+                    const cx<R> pe = ldc<true>(A.p + e);
+                    const cx<double> pd = {(double)pe.x, (double)pe.y};
+                    stc<true>(A.x + e, ldc<true>(A.x + e) + alpha * pd);
+                    const cx<R> rn = ldc<true>(A.r + e) - alpha_r * ddp;
                     stc<true>(A.r + e, rn);
-                    nrm += norm2(rn);
+                    nrm += (double)norm2(rn);
                 }
         }
     }
@@ -181,8 +184,8 @@ static hipError_t launch_t(const Geom &g, const DslashLaunch &L, hipStream_t s) 
     A.out_par = L.out_par;
     A.done = L.done;
     A.x = (cx<double> *)L.x;
-    A.p = (const cx<double> *)L.p;
-    A.r = (cx<double> *)L.r;
+    A.p = (const cx<R> *)L.p;
+    A.r = (cx<R> *)L.r;
     A.red = L.red;
     A.fin = L.fin;
     const int blk = g.blk;
@@ -211,7 +214,9 @@ hipError_t launch_dslash(int prec, const Geom &g, const DslashLaunch &L, hipStre
         return L.sign > 0 ? launch_epi<double, 1>(g, L, s) : launch_epi<double, -1>(g, L, s);
     }
     if (prec == WL_F32) {
-        if (L.epi == EPI_CGUPD) return hipErrorInvalidValue;
+        if (L.epi == EPI_CGUPD)
+            return L.sign > 0 ? launch_t<float, 1, EPI_CGUPD>(g, L, s)
+                              : launch_t<float, -1, EPI_CGUPD>(g, L, s);
         return L.sign > 0 ? launch_epi<float, 1>(g, L, s) : launch_epi<float, -1>(g, L, s);
     }
     return hipErrorInvalidValue;

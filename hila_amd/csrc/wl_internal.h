@@ -66,9 +66,12 @@ hipError_t launch_norm2(int prec, const Geom &g, const void *f, const RedArgs &r
 // r = b - DDp; p = r; reduce ||r||^2 -> FIN_RR   (conjugate_gradient.h:67-72)
 hipError_t launch_cg_init(const Geom &g, const void *b, const void *DDp, void *r, void *p,
                           const RedArgs &ra, hipStream_t s);
-// p = beta p + r, beta from the CG state (conjugate_gradient.h:93-94)
-hipError_t launch_cg_xpay(const Geom &g, void *p, const void *r, const CgState *st,
-                          const int *done, hipStream_t s);
+// p = beta p + r, beta from the CG state (conjugate_gradient.h:93-94); skipped when done != 0
+hipError_t launch_cg_xpay(int prec, const Geom &g, void *p, const void *r, const CgState *st,
+                          hipStream_t s);
+// reliable update: r_lo = b - DDp (FP64 in, inner precision out), ||.||^2 -> FIN_UPDATE
+hipError_t launch_residual_update(int prec, const Geom &g, const void *b, const void *DDp,
+                                  void *r_lo, const RedArgs &ra, hipStream_t s);
 // reduce ||b - v||^2 with `fin`
 hipError_t launch_diff_norm2(const Geom &g, const void *b, const void *v, const RedArgs &ra,
                              int fin, hipStream_t s);
@@ -119,6 +122,7 @@ struct wl_ctx {
     wl_field *tmp_odd64 = nullptr;   // odd temporary of M
     wl_field *tmp_odd32 = nullptr;
     wl_field *cg_r = nullptr, *cg_p = nullptr, *cg_Dp = nullptr, *cg_DDp = nullptr;
+    wl_field *mx_r = nullptr, *mx_p = nullptr, *mx_Dp = nullptr;  // inner-precision Krylov vectors
     // fields created through wl_field_create and not yet destroyed; wl_destroy orphans them
     std::unordered_set<wl_field *> user_fields;
     std::mutex mu;

@@ -33,7 +33,7 @@ EXPORTED_SYMBOLS = [
     "wl_version", "wl_last_error", "wl_create", "wl_destroy", "wl_sync", "wl_volume",
     "wl_gauge_upload", "wl_field_create", "wl_field_destroy", "wl_field_upload",
     "wl_field_download", "wl_field_zero", "wl_field_copy", "wl_field_norm2", "wl_dslash_eo", "wl_meo_apply",
-    "wl_cg_solve", "wl_set_option", "wl_get_option", "wl_time_op", "wl_gen_gauge", "wl_gen_spinor", "wl_gen_point",
+    "wl_cg_solve", "wl_cg_solve_mixed", "wl_set_option", "wl_get_option", "wl_time_op", "wl_gen_gauge", "wl_gen_spinor", "wl_gen_point",
 ]
 
 
@@ -82,6 +82,7 @@ def lib():
         L.wl_dslash_eo.argtypes = [vp, vp, vp, d, i]
         L.wl_meo_apply.argtypes = [vp, vp, vp, d, i]
         L.wl_cg_solve.argtypes = [vp, vp, vp, d, d, i, C.POINTER(CGStats)]
+        L.wl_cg_solve_mixed.argtypes = [vp, vp, vp, d, d, i, i, d, i, C.POINTER(CGStats)]
         L.wl_set_option.argtypes = [vp, i, i]
         L.wl_get_option.argtypes = [vp, i, C.POINTER(i)]
         L.wl_time_op.argtypes = [vp, i, vp, vp, d, i, i, i, C.POINTER(d)]
@@ -301,18 +302,33 @@ class Dirac_Wilson_evenodd:
 
 
 class CG:
-    """HILA's CG<Op> (conjugate_gradient.h:19-105): solves (M^dag M) out = in."""
+    """HILA's CG<Op> (conjugate_gradient.h:19-105): solves (M^dag M) out = in.
 
-    def __init__(self, op: Dirac_Wilson_evenodd, accuracy=1e-12, maxiters=10000):
+    precision="f64" runs HILA's algorithm in FP64.  precision="f32" runs the mixed-precision
+    solver with reliable updates (inner operator and Krylov vectors in FP32, solution and
+    reductions in FP64, FP64 true residual recomputed at every update and at the end);
+    delta and update_every set when reliable updates happen."""
+
+    def __init__(self, op: Dirac_Wilson_evenodd, accuracy=1e-12, maxiters=10000,
+                 precision="f64", delta=1e-3, update_every=100):
         self.M = op
         self.accuracy = float(accuracy)
         self.maxiters = int(maxiters)
+        self.precision = precision
+        self.delta = float(delta)
+        self.update_every = int(update_every)
 
     def apply(self, fin: Field, fout: Field, raise_on_noconv=False):
         """Runs CG; fout holds the initial guess on entry.  Returns per-rhs stats dicts.
         Like HILA, reaching maxiters is not an error unless raise_on_noconv is set."""
         stats = (CGStats * fin.nrhs)()
-        rc = lib().wl_cg_solve(self.M.lat.handle, fout.handle, fin.handle, self.M.kappa,
-                               self.accuracy, self.maxiters, stats)
+        if self.precision == "f64":
+            rc = lib().wl_cg_solve(self.M.lat.handle, fout.handle, fin.handle, self.M.kappa,
+                                   self.accuracy, self.maxiters, stats)
+        else:
+            inner = {"f32": F32, "f64-mixed": F64}[self.precision]
+            rc = lib().wl_cg_solve_mixed(self.M.lat.handle, fout.handle, fin.handle,
+                                         self.M.kappa, self.accuracy, self.maxiters, inner,
+                                         self.delta, self.update_every, stats)
         check(rc, allow=() if raise_on_noconv else (WL_E_NOCONV,))
         return [s.as_dict() for s in stats]

@@ -144,6 +144,19 @@ int wl_meo_apply(wl_ctx *ctx, wl_field *out, const wl_field *in, double kappa, i
 int wl_cg_solve(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa, double accuracy,
                 int maxiters, wl_cg_stats *stats);
 
+/* Mixed-precision CG on M^dag M with reliable updates (an addition: HILA's only mixed
+ * precision is a full FP32 CG used as the initial guess of an FP64 CG,
+ * libraries/hmc/fermion_field.h:65-86).  The operator and the Krylov vectors r, p, Dp are in
+ * inner_prec (WL_F32; WL_F64 runs the same algorithm in FP64); the solution x and every dot /
+ * norm reduction are FP64.  A reliable update -- r = b - M^dag M x with the FP64 operator --
+ * is made when ||r|| has dropped by the factor delta since the last one, every update_every
+ * iterations, and whenever the iterated residual meets the target; convergence is declared
+ * only on that FP64 true residual: ||b - M^dag M x|| < accuracy ||b||.  x is the initial guess
+ * on entry.  Same return convention as wl_cg_solve. */
+int wl_cg_solve_mixed(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa, double accuracy,
+                      int maxiters, int inner_prec, double delta, int update_every,
+                      wl_cg_stats *stats);
+
 /* ---- kernel options ------------------------------------------------------------------- */
 
 /* Tuning knobs of the hopping kernel (no counterpart in HILA, whose GPU loops are generated
