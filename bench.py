@@ -34,6 +34,75 @@ METRIC = "Dslash GB/s & %HBM roofline; CG solves/s + p99 latency @1e-10 true res
 PMC_FILE = os.path.join(ROOT, "profiles", "dslash_f64_32x32x32x32_pmc.json")
 
 
+def rel_diff(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def solve_series(W, D, fin, fx, n, precision, accuracy=1e-10):
+    """n solves of M^dag M x = b from x = 0; latency percentiles, iterations, true residuals."""
+    cg = W.CG(D, accuracy, 10000, precision=precision)
+    lat_ms, iters, tres, upd = [], [], [], []
+    for i in range(n):
+        fx.zero()
+        t = time.perf_counter()
+        st = cg.apply(fin, fx)[0]
+        lat_ms.append(1e3 * (time.perf_counter() - t))
+        iters.append(st["iterations"])
+        tres.append(st["true_resid"])
+        upd.append(st["reliable_updates"])
+        print(f"# cg {precision} solve {i}: {lat_ms[-1]:.1f} ms, {st['iterations']} it, "
+              f"{st['reliable_updates']} updates, true resid {st['true_resid']:.2e}",
+              file=sys.stderr, flush=True)
+        if not st["converged"]:
+            raise RuntimeError(f"CG ({precision}) did not converge: {st}")
+    if not lat_ms:
+        return None
+    arr = np.array(lat_ms)
+    return {
+        "solves_per_s": round(1e3 / arr.mean(), 4),
+        "p50_ms": round(float(np.percentile(arr, 50)), 2),
+        "p99_ms": round(float(np.percentile(arr, 99)), 2),
+        "iterations": int(np.median(iters)),
+        "reliable_updates": int(np.median(upd)),
+        "max_true_resid": float(max(tres)),
+        "ms_per_iteration": round(float(arr.mean()) / float(np.mean(iters)), 4),
+        "precision": precision,
+        "solves": len(lat_ms),
+    }
+
+
+def mixed_precision_config(W, n):
+    """Mixed-precision configuration: 48^3x96, links exp(i 0.3 H) (seed 6), kappa 0.1245,
+    Gaussian source (seed 7), CG to 1e-10 true residual in FP64 and with the FP32 inner solver;
+    the mixed solution's error is stated against the FP64 CG solution."""
+    dims, kappa = (48, 48, 48, 96), 0.1245
+    t0 = time.perf_counter()
+    U = W.gen_gauge(dims, "smooth", 0.3, 6)
+    lat = W.Lattice(dims)
+    lat.upload_gauge(U)
+    del U
+    b = W.Field(lat).upload(W.gen_spinor(dims, 7, W.EVEN))
+    D = W.Dirac_Wilson_evenodd(kappa, lat)
+    print(f"# 48^3x96 setup {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    fo64, fo32 = W.Field(lat, W.ODD), W.Field(lat, W.ODD, W.F32)
+    fin32 = W.Field(lat, W.EVEN, W.F32).upload(b.download())
+    us64 = 1e3 * D.time(b, fo64, "hop", 1, warmup=3, iters=20)
+    us32 = 1e3 * D.time(fin32, fo32, "hop", 1, warmup=3, iters=20)
+    del fo64, fo32, fin32
+    x64, x32 = W.Field(lat), W.Field(lat)
+    r64 = solve_series(W, D, b, x64, n, "f64")
+    r32 = solve_series(W, D, b, x32, n, "f32")
+    out = {"lattice": list(dims), "kappa": kappa, "f64": r64, "f32_mixed": r32,
+           "speedup_vs_f64": round(r32["solves_per_s"] / r64["solves_per_s"], 3),
+           "solution_rel_diff_vs_f64": rel_diff(x32.download(), x64.download()),
+           "dslash_f64_us": round(us64, 1),
+           "dslash_f64_hbm_frac": round(1536 * lat.Vh / (us64 * 1e-6) / 8e12, 4),
+           "dslash_f32_us": round(us32, 1),
+           "dslash_f32_hbm_frac": round(768 * lat.Vh / (us32 * 1e-6) / 8e12, 4)}
+    lat.close()
+    return out
+
+
 def cpu_baseline(U, psi, seconds=15.0):
     """The oracle's HILA-structured hopping term (projection pass + per-direction
     accumulation, wilson.h:53-95) on the host cores, on a bounded number of applications."""
@@ -67,6 +136,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cg-solves", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-big", action="store_true",
+                    help="skip the 48^3x96 mixed-precision configuration")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: --steps hopping launches, then --steps "
                          "plain field copies (12 planes of V/2 complex doubles each way) as "
@@ -128,32 +199,20 @@ def main(argv=None):
     print(f"# dslash {ms_kernel * 1e3:.1f} us/launch {gbs_kernel:.0f} GB/s", file=sys.stderr,
           flush=True)
 
-    # --- CG solves to 1e-10 ---
-    cg = W.CG(D, 1e-10, 10000)
-    fx = W.Field(lat, W.EVEN)
-    lat_ms, iters, tres = [], [], []
-    for i in range(a.cg_solves):
-        fx.zero()
-        t = time.perf_counter()
-        st = cg.apply(fin, fx)[0]
-        lat_ms.append(1e3 * (time.perf_counter() - t))
-        iters.append(st["iterations"])
-        tres.append(st["true_resid"])
-        print(f"# cg solve {i}: {lat_ms[-1]:.1f} ms, {st['iterations']} it, "
-              f"true resid {st['true_resid']:.2e}", file=sys.stderr, flush=True)
-    cg_out = None
-    if lat_ms:
-        arr = np.array(lat_ms)
-        cg_out = {
-            "solves_per_s": round(1e3 / arr.mean(), 4),
-            "p50_ms": round(float(np.percentile(arr, 50)), 2),
-            "p99_ms": round(float(np.percentile(arr, 99)), 2),
-            "iterations": int(np.median(iters)),
-            "max_true_resid": float(max(tres)),
-            "ms_per_iteration": round(float(arr.mean()) / float(np.mean(iters)), 4),
-            "precision": "f64",
-            "solves": len(lat_ms),
-        }
+    # --- CG solves to 1e-10: FP64 (HILA's algorithm) and mixed FP32 with reliable updates ---
+    x64 = W.Field(lat, W.EVEN)
+    cg_out = solve_series(W, D, fin, x64, a.cg_solves, "f64")
+    xmx = W.Field(lat, W.EVEN)
+    cg_mixed = solve_series(W, D, fin, xmx, a.cg_solves, "f32")
+    if cg_out and cg_mixed:
+        cg_mixed["speedup_vs_f64"] = round(cg_mixed["solves_per_s"] / cg_out["solves_per_s"], 3)
+        cg_mixed["solution_rel_diff_vs_f64"] = rel_diff(xmx.download(), x64.download())
+    del x64, xmx
+    big = None
+    if a.cg_solves > 0 and not a.no_big:
+        D = fin = fodd = None
+        lat.close()
+        big = mixed_precision_config(W, max(2, a.cg_solves // 2))
 
     traffic = None
     if os.path.exists(PMC_FILE):
@@ -184,6 +243,8 @@ def main(argv=None):
         "gflops": round(FLOPS_PER_SITE * vh / (ms_kernel * 1e-3) / 1e9, 1),
         "meo_us": round(ms_M * 1e3, 2),
         "cg": cg_out,
+        "cg_mixed_f32": cg_mixed,
+        "mixed_48x48x48x96": big,
         "cpu_baseline": None,
     }
     if not a.no_cpu:
