@@ -69,6 +69,11 @@ int field_new(wl_ctx *c, int parity, int prec, int nrhs, wl_field **out) {
     f->nrhs = nrhs;
     f->rhs_elems = (size_t)12 * c->g.stride;
     f->bytes = f->rhs_elems * nrhs * prec_bytes(prec);
+    if (f->rhs_elems * (size_t)nrhs >= ((size_t)1 << 31)) {
+        delete f;
+        return fail(WL_E_ARG, "%d right-hand sides exceed 2^31 elements per field at this volume",
+                    nrhs);
+    }
     if (hipMalloc(&f->data, f->bytes) != hipSuccess) {
         const size_t want = f->bytes;
         delete f;
@@ -152,6 +157,98 @@ int meo_one(wl_ctx *c, int prec, void *out, const void *in, void *tmp, double ka
 
 int check_sign(int sign) { return (sign == 1 || sign == -1) ? WL_OK : fail(WL_E_ARG, "sign must be +-1"); }
 
+int ensure_partials(wl_ctx *c, size_t n) {
+    if (c->partials_cap >= n) return WL_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(c->partials));
+    c->partials = nullptr;
+    c->partials_cap = 0;
+    if (hipMalloc(&c->partials, n * sizeof(double)) != hipSuccess)
+        return fail(WL_E_OOM, "reduction buffer of %zu doubles", n);
+    c->partials_cap = n;
+    return WL_OK;
+}
+
+int ensure_st_mr(wl_ctx *c, int N) {
+    if (c->st_mr_cap >= N) return WL_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->st_mr) (void)hipFree(c->st_mr);
+    if (c->st_mr_host) (void)hipHostFree(c->st_mr_host);
+    c->st_mr = nullptr;
+    c->st_mr_host = nullptr;
+    c->st_mr_cap = 0;
+    if (hipMalloc(&c->st_mr, N * sizeof(CgState)) != hipSuccess ||
+        hipHostMalloc(&c->st_mr_host, N * sizeof(CgState), hipHostMallocDefault) != hipSuccess)
+        return fail(WL_E_OOM, "CG states for %d right-hand sides", N);
+    c->st_mr_cap = N;
+    return WL_OK;
+}
+
+// Work field `slot` of the given precision, parity and rhs count (created on first use, kept
+// resident with the context).
+enum WorkSlot { W_R = 0, W_P, W_DP, W_T, W_DP64, W_DDP64, W_T64, W_X };
+int work_field(wl_ctx *c, int slot, int prec, int parity, int nrhs, wl_field **out) {
+    const auto key = std::make_tuple(slot, prec, parity, nrhs);
+    auto it = c->work.find(key);
+    if (it != c->work.end()) {
+        *out = it->second;
+        return WL_OK;
+    }
+    int rc = field_new(c, parity, prec, nrhs, out);
+    if (rc == WL_OK) c->work[key] = *out;
+    return rc;
+}
+
+// Right-hand sides per lane of the multi-rhs kernel (register blocking).
+int rhs_per_lane(const wl_ctx *c, int prec, int N) {
+    const int cap = prec == WL_F64 ? 2 : 4;
+    const int nr = c->rhs_block > 0 ? c->rhs_block : cap;
+    return std::max(1, std::min(nr, std::min(N, cap)));
+}
+
+// out = a H_sign in (+ xin, fused CG update, ...) for all N right-hand sides in one launch;
+// per-rhs partial sums land in c->partials rows, mr_segments apart.
+int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, const void *xin,
+                  double a, int sign, int epi, int N, const int *all_done, CgState *st, void *x,
+                  const void *p, void *r, int *pstride_out) {
+    const int pstride = mr_segments(c->g);
+    if (epi == EPI_AXPY_NORM || epi == EPI_CGUPD) {
+        int rc = ensure_partials(c, (size_t)N * pstride);
+        if (rc) return rc;
+    }
+    MrLaunch L{};
+    L.out = out;
+    L.in = in;
+    L.xin = xin;
+    L.gauge = gauge_for(c, prec);
+    L.a = a;
+    L.out_par = out_par;
+    L.sign = sign;
+    L.epi = epi;
+    L.j0 = 0;
+    L.NB = N;
+    L.NR = rhs_per_lane(c, prec, N);
+    L.all_done = all_done;
+    L.st = st;
+    L.x = x;
+    L.p = p;
+    L.r = r;
+    L.partials = c->partials;
+    HIPCHK(launch_dslash_mr(prec, c->g, L, c->stream));
+    if (pstride_out) *pstride_out = pstride;
+    return WL_OK;
+}
+
+// M_s for N right-hand sides (even fields), through the odd work field t.
+int meo_mr(wl_ctx *c, int prec, void *out, const void *in, void *t, double kappa, int sign,
+           int epi2, int N, const int *all_done, CgState *st, int *pstride_out) {
+    int rc = dslash_mr_all(c, prec, t, 1, in, nullptr, -kappa, sign, EPI_SET, N, all_done, st,
+                           nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    return dslash_mr_all(c, prec, out, 0, t, in, kappa, sign, epi2, N, all_done, st, nullptr,
+                         nullptr, nullptr, pstride_out);
+}
+
 // Segments (64 checkerboard sites) per time slice, or 0 when a slice is not a whole number of
 // segments (then only the memory-order traversal is available).
 int segments_per_slice(const Geom &g) {
@@ -220,12 +317,15 @@ extern "C" int wl_create(const int dims[4], int antiperiodic_t, int device, wl_c
     const int nb = (c->g.vh + 63) / 64;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->partials, sizeof(double) * (nb + 64)) != hipSuccess ||
+        hipMalloc(&c->flags_d, 64) != hipSuccess ||
+        hipHostMalloc(&c->flags_h, 64, hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&c->st, sizeof(CgState)) != hipSuccess ||
         hipMalloc(&c->scratch_d, 256) != hipSuccess ||
         hipHostMalloc(&c->st_host, sizeof(CgState), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&c->scratch_h, 256, hipHostMallocDefault) != hipSuccess) {
         rc = fail(WL_E_DEVICE, "context allocation failed");
     } else if (hipMemsetAsync(c->st, 0, sizeof(CgState), c->stream) != hipSuccess ||
+               hipMemsetAsync(c->flags_d, 0, 64, c->stream) != hipSuccess ||
                hipStreamSynchronize(c->stream) != hipSuccess) {
         rc = fail(WL_E_DEVICE, "context initialisation failed");
     }
@@ -233,6 +333,7 @@ extern "C" int wl_create(const int dims[4], int antiperiodic_t, int device, wl_c
         wl_destroy(c);
         return rc;
     }
+    c->partials_cap = nb + 64;
     *out = c;
     return WL_OK;
 }
@@ -244,6 +345,12 @@ extern "C" int wl_destroy(wl_ctx *c) {
     for (wl_field *f : {c->tmp_odd64, c->tmp_odd32, c->cg_r, c->cg_p, c->cg_Dp, c->cg_DDp,
                         c->mx_r, c->mx_p, c->mx_Dp})
         field_free(f);
+    for (auto &kv : c->work) field_free(kv.second);
+    c->work.clear();
+    if (c->st_mr) (void)hipFree(c->st_mr);
+    if (c->st_mr_host) (void)hipHostFree(c->st_mr_host);
+    if (c->flags_d) (void)hipFree(c->flags_d);
+    if (c->flags_h) (void)hipHostFree(c->flags_h);
     // fields the caller still holds: release their device memory and orphan the handles
     for (wl_field *f : c->user_fields) {
         if (f->data) (void)hipFree(f->data);
@@ -334,7 +441,7 @@ extern "C" int wl_field_upload(wl_field *f, int rhs, const double *host) {
     int rc = ensure_staging(c, bytes);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(c->staging, host, bytes, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_aos_to_soa(f->prec, c->g, (const double *)c->staging, f->rhs_ptr(rhs), c->stream));
+    HIPCHK(launch_aos_to_soa(f->prec, c->g, (const double *)c->staging, f->data, f->nrhs, rhs, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return WL_OK;
 }
@@ -349,7 +456,7 @@ extern "C" int wl_field_download(const wl_field *f, int rhs, double *host) {
     const size_t bytes = (size_t)c->g.vh * 24 * sizeof(double);
     int rc = ensure_staging(c, bytes);
     if (rc) return rc;
-    HIPCHK(launch_soa_to_aos(f->prec, c->g, f->rhs_ptr(rhs), (double *)c->staging, c->stream));
+    HIPCHK(launch_soa_to_aos(f->prec, c->g, f->data, (double *)c->staging, f->nrhs, rhs, c->stream));
     HIPCHK(hipMemcpyAsync(host, c->staging, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return WL_OK;
@@ -370,7 +477,7 @@ extern "C" int wl_field_norm2(const wl_field *f, int rhs, double *out) {
     wl_ctx *c = f->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
     DeviceGuard dg(c->device);
-    HIPCHK(launch_norm2(f->prec, c->g, f->rhs_ptr(rhs), red_args(c), FIN_STORE, nullptr, c->stream));
+    HIPCHK(launch_norm2(f->prec, c->g, f->data, f->nrhs, rhs, red_args(c), FIN_STORE, nullptr, c->stream));
     HIPCHK(hipMemcpyAsync(c->scratch_h, c->scratch_d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     *out = c->scratch_h[0];
@@ -391,12 +498,11 @@ extern "C" int wl_dslash_eo(wl_ctx *c, wl_field *out, const wl_field *in, double
     int rc = ensure_gauge(c, in->prec);
     if (rc) return rc;
     const int out_par = out->parity == WL_EVEN ? 0 : 1;
-    for (int r = 0; r < in->nrhs; ++r) {
-        rc = dslash_one(c, in->prec, out->rhs_ptr(r), out_par, in->rhs_ptr(r), nullptr, -kappa,
-                        sign, EPI_SET, nullptr, FIN_NONE);
-        if (rc) return rc;
-    }
-    return WL_OK;
+    if (in->nrhs == 1)
+        return dslash_one(c, in->prec, out->data, out_par, in->data, nullptr, -kappa, sign, EPI_SET,
+                          nullptr, FIN_NONE);
+    return dslash_mr_all(c, in->prec, out->data, out_par, in->data, nullptr, -kappa, sign, EPI_SET,
+                         in->nrhs, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 extern "C" int wl_meo_apply(wl_ctx *c, wl_field *out, const wl_field *in, double kappa, int sign) {
@@ -411,15 +517,17 @@ extern "C" int wl_meo_apply(wl_ctx *c, wl_field *out, const wl_field *in, double
     DeviceGuard dg(c->device);
     int rc = ensure_gauge(c, in->prec);
     if (rc) return rc;
-    wl_field **tmp = in->prec == WL_F64 ? &c->tmp_odd64 : &c->tmp_odd32;
-    rc = ensure_tmp(c, in->prec, tmp, WL_ODD);
-    if (rc) return rc;
-    for (int r = 0; r < in->nrhs; ++r) {
-        rc = meo_one(c, in->prec, out->rhs_ptr(r), in->rhs_ptr(r), (*tmp)->data, kappa, sign,
-                     EPI_AXPY, nullptr, FIN_NONE);
+    if (in->nrhs == 1) {
+        wl_field **tmp = in->prec == WL_F64 ? &c->tmp_odd64 : &c->tmp_odd32;
+        rc = ensure_tmp(c, in->prec, tmp, WL_ODD);
         if (rc) return rc;
+        return meo_one(c, in->prec, out->data, in->data, (*tmp)->data, kappa, sign, EPI_AXPY,
+                       nullptr, FIN_NONE);
     }
-    return WL_OK;
+    wl_field *t;
+    if ((rc = work_field(c, W_T, in->prec, WL_ODD, in->nrhs, &t))) return rc;
+    return meo_mr(c, in->prec, out->data, in->data, t->data, kappa, sign, EPI_AXPY, in->nrhs,
+                  nullptr, nullptr, nullptr);
 }
 
 // =================================================================== CG =================
@@ -447,7 +555,7 @@ int cg_f64(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int
     *c->st_host = init;
     HIPCHK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
     // source_norm and target_rr (conjugate_gradient.h:61-63)
-    HIPCHK(launch_norm2(WL_F64, c->g, b, ra, FIN_SOURCE, nullptr, c->stream));
+    HIPCHK(launch_norm2(WL_F64, c->g, b, 1, 0, ra, FIN_SOURCE, nullptr, c->stream));
     // r = b - M^dag M x ; p = r ; rr = |r|^2  (:65-73)
     if ((rc = meo_one(c, WL_F64, Dp, x, t, kappa, +1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
     if ((rc = meo_one(c, WL_F64, DDp, Dp, t, kappa, -1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
@@ -570,7 +678,7 @@ int cg_mixed(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, i
     init.update_every = update_every;
     *c->st_host = init;
     HIPCHK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_norm2(WL_F64, c->g, b, ra, FIN_SOURCE, nullptr, c->stream));
+    HIPCHK(launch_norm2(WL_F64, c->g, b, 1, 0, ra, FIN_SOURCE, nullptr, c->stream));
 
     // reliable update: r = b - M^dag M x in FP64, r_lo = r, restart (or converged)
     auto reliable_update = [&]() -> int {
@@ -581,7 +689,7 @@ int cg_mixed(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, i
         return WL_OK;
     };
     if ((rc = reliable_update())) return rc;
-    HIPCHK(launch_copy(prec, c->g, r, p, c->stream));  // p = r
+    HIPCHK(launch_copy(prec, c->g, r, p, 1, c->stream));  // p = r
     HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->st_host->source_norm == 0.0) {
@@ -652,6 +760,139 @@ int cg_mixed(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, i
     return WL_OK;
 }
 
+// Batched CG for N right-hand sides that share every operator application: one CgState per
+// rhs, each with HILA's recurrence (conjugate_gradient.h:75-95).  mixed = false: FP64
+// throughout, each rhs stops at its own iterated criterion exactly as HILA's CG.  mixed = true:
+// inner operator and Krylov vectors in `prec`, FP64 solution and reductions, reliable updates
+// (recomputed FP64 residual) taken by all running rhs together whenever one of them asks
+// (residual drop by delta, update_every iterations, or iterated convergence), convergence on
+// the FP64 true residual.
+int cg_batched(wl_ctx *c, void *x, const void *b, int N, double kappa, double accuracy,
+               int maxiters, int prec, bool mixed, double delta, int update_every,
+               wl_cg_stats *stats) {
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    if ((rc = ensure_gauge(c, prec))) return rc;
+    if ((rc = ensure_st_mr(c, N))) return rc;
+    wl_field *R, *P, *DP, *T, *DP64, *DDP64, *T64;
+    if ((rc = work_field(c, W_R, prec, WL_EVEN, N, &R)) || (rc = work_field(c, W_P, prec, WL_EVEN, N, &P)) ||
+        (rc = work_field(c, W_DP, prec, WL_EVEN, N, &DP)) || (rc = work_field(c, W_T, prec, WL_ODD, N, &T)) ||
+        (rc = work_field(c, W_DP64, WL_F64, WL_EVEN, N, &DP64)) ||
+        (rc = work_field(c, W_DDP64, WL_F64, WL_EVEN, N, &DDP64)) ||
+        (rc = work_field(c, W_T64, WL_F64, WL_ODD, N, &T64)))
+        return rc;
+    CgState *st = c->st_mr;
+    const int *all_done = c->flags_d;
+    for (int j = 0; j < N; ++j) {
+        CgState s0{};
+        s0.acc2 = accuracy * accuracy;
+        s0.maxiters = maxiters;
+        s0.delta2 = delta * delta;
+        s0.update_every = update_every;
+        c->st_mr_host[j] = s0;
+    }
+    HIPCHK(hipMemcpyAsync(st, c->st_mr_host, N * sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->flags_d, 0, 64, c->stream));
+    const int pplain = nblocks(c->g);
+    if ((rc = ensure_partials(c, (size_t)N * pplain))) return rc;
+    HIPCHK(launch_norm2_mr(WL_F64, c->g, b, N, c->partials, c->stream));
+    HIPCHK(launch_finish_mr(c->partials, pplain, pplain, N, st, nullptr, FIN_SOURCE, nullptr, c->stream));
+
+    // r = b - M^dag M x, p = r
+    if ((rc = meo_mr(c, WL_F64, DP64->data, x, T64->data, kappa, +1, EPI_AXPY, N, nullptr, nullptr, nullptr)))
+        return rc;
+    if ((rc = meo_mr(c, WL_F64, DDP64->data, DP64->data, T64->data, kappa, -1, EPI_AXPY, N, nullptr, nullptr, nullptr)))
+        return rc;
+    if (!mixed) {
+        HIPCHK(launch_cg_init_mr(c->g, b, DDP64->data, R->data, P->data, N, c->partials, c->stream));
+        HIPCHK(launch_finish_mr(c->partials, pplain, pplain, N, st, nullptr, FIN_RR, nullptr, c->stream));
+    } else {
+        HIPCHK(launch_residual_update_mr(prec, c->g, b, DDP64->data, R->data, st, 1, N, c->partials, c->stream));
+        HIPCHK(launch_finish_mr(c->partials, pplain, pplain, N, st, nullptr, FIN_UPDATE_FIRST, nullptr, c->stream));
+        HIPCHK(launch_copy(prec, c->g, R->data, P->data, N, c->stream));
+    }
+    HIPCHK(launch_batch_sync(st, N, 0, c->flags_d, c->stream));
+    if (maxiters <= 0) {
+        for (int j = 0; j < N; ++j) c->st_mr_host[j].done = 2;
+    }
+
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    int enqueued = 0;
+    const int cap = maxiters + 2 * kChunk + 16;
+    while (maxiters > 0) {
+        for (int it = 0; it < kChunk; ++it) {
+            int ps = 0;
+            if ((rc = meo_mr(c, prec, DP->data, P->data, T->data, kappa, +1, EPI_AXPY_NORM, N, all_done, st, &ps)))
+                return rc;
+            HIPCHK(launch_finish_mr(c->partials, ps, ps, N, st, nullptr, FIN_ALPHA, all_done, c->stream));
+            if ((rc = dslash_mr_all(c, prec, T->data, 1, DP->data, nullptr, -kappa, -1, EPI_SET, N, all_done, st,
+                                    nullptr, nullptr, nullptr, nullptr)))
+                return rc;
+            if ((rc = dslash_mr_all(c, prec, nullptr, 0, T->data, DP->data, kappa, -1, EPI_CGUPD, N, all_done, st,
+                                    x, P->data, R->data, &ps)))
+                return rc;
+            HIPCHK(launch_finish_mr(c->partials, ps, ps, N, st, nullptr, mixed ? FIN_RRNEW_MIXED2 : FIN_RRNEW,
+                                    all_done, c->stream));
+            HIPCHK(launch_batch_sync(st, N, mixed ? 1 : 0, c->flags_d, c->stream));
+            HIPCHK(launch_xpay_mr(prec, c->g, P->data, R->data, st, all_done, N, c->stream));
+        }
+        enqueued += kChunk;
+        HIPCHK(hipMemcpyAsync(c->flags_h, c->flags_d, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipEventRecord(ev, c->stream));
+        HIPCHK(hipEventSynchronize(ev));
+        if (c->flags_h[0]) break;
+        if (mixed && c->flags_h[1]) {  // reliable update of every running rhs
+            if ((rc = meo_mr(c, WL_F64, DP64->data, x, T64->data, kappa, +1, EPI_AXPY, N, nullptr, nullptr, nullptr)))
+                return rc;
+            if ((rc = meo_mr(c, WL_F64, DDP64->data, DP64->data, T64->data, kappa, -1, EPI_AXPY, N, nullptr, nullptr,
+                             nullptr)))
+                return rc;
+            HIPCHK(launch_residual_update_mr(prec, c->g, b, DDP64->data, R->data, st, 0, N, c->partials,
+                                             c->stream));
+            HIPCHK(launch_finish_mr(c->partials, pplain, pplain, N, st, nullptr, FIN_UPDATE, nullptr, c->stream));
+            HIPCHK(launch_xpay_mr(prec, c->g, P->data, R->data, st, nullptr, N, c->stream));
+            HIPCHK(launch_batch_sync(st, N, 0, c->flags_d, c->stream));
+        }
+        if (enqueued > cap) break;  // every rhs stops at maxiters; this is a guard only
+    }
+    (void)hipEventDestroy(ev);
+
+    // FP64 true residual of every rhs
+    if ((rc = meo_mr(c, WL_F64, DP64->data, x, T64->data, kappa, +1, EPI_AXPY, N, nullptr, nullptr, nullptr)))
+        return rc;
+    if ((rc = meo_mr(c, WL_F64, DDP64->data, DP64->data, T64->data, kappa, -1, EPI_AXPY, N, nullptr, nullptr, nullptr)))
+        return rc;
+    HIPCHK(launch_residual_update_mr(WL_F64, c->g, b, DDP64->data, DP64->data, st, 1, N, c->partials,
+                                     c->stream));
+    HIPCHK(launch_finish_mr(c->partials, pplain, pplain, N, st, nullptr, FIN_TRUE, nullptr, c->stream));
+    HIPCHK(hipMemcpyAsync(c->st_mr_host, st, N * sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int worst = WL_OK;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (int j = 0; j < N; ++j) {
+        const CgState &S = c->st_mr_host[j];
+        if (S.source_norm == 0.0) {  // b_j = 0: x_j = 0
+            HIPCHK(launch_zero_rhs(c->g, x, j, c->stream));
+        }
+        const bool conv = S.done == 1;
+        if (!conv) worst = WL_E_NOCONV;
+        if (stats) {
+            wl_cg_stats &o = stats[j];
+            std::memset(&o, 0, sizeof(o));
+            o.iterations = S.iter;
+            o.converged = conv ? 1 : 0;
+            o.rr_rel = S.source_norm > 0 ? S.rrnew / S.source_norm : 0.0;
+            o.true_resid = S.source_norm > 0 ? std::sqrt(S.true_rr / S.source_norm) : 0.0;
+            o.reliable_updates = mixed ? std::max(0, S.nreliable - 1) : 0;
+            o.ms = ms;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (worst) return fail(WL_E_NOCONV, "batched CG: not every right-hand side converged");
+    return WL_OK;
+}
+
 }  // namespace
 
 extern "C" int wl_cg_solve(wl_ctx *c, wl_field *x, const wl_field *b, double kappa, double accuracy,
@@ -667,15 +908,9 @@ extern "C" int wl_cg_solve(wl_ctx *c, wl_field *x, const wl_field *b, double kap
     DeviceGuard dg(c->device);
     int rc = ensure_gauge(c, WL_F64);
     if (rc) return rc;
-    int worst = WL_OK;
-    for (int r = 0; r < b->nrhs; ++r) {
-        wl_cg_stats s{};
-        rc = cg_f64(c, x->rhs_ptr(r), b->rhs_ptr(r), kappa, accuracy, maxiters, &s);
-        if (rc && rc != WL_E_NOCONV) return rc;
-        if (rc) worst = rc;
-        if (stats) stats[r] = s;
-    }
-    return worst;
+    if (b->nrhs == 1) return cg_f64(c, x->data, b->data, kappa, accuracy, maxiters, stats);
+    return cg_batched(c, x->data, b->data, b->nrhs, kappa, accuracy, maxiters, WL_F64, false, 0.1,
+                      1 << 30, stats);
 }
 
 // ================================================================ diagnostics ===========
@@ -721,8 +956,7 @@ extern "C" int wl_field_copy(wl_field *dst, const wl_field *src) {
     wl_ctx *c = dst->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
     DeviceGuard dg(c->device);
-    for (int r = 0; r < src->nrhs; ++r)
-        HIPCHK(launch_copy(src->prec, c->g, src->rhs_ptr(r), dst->rhs_ptr(r), c->stream));
+    HIPCHK(launch_copy(src->prec, c->g, src->data, dst->data, src->nrhs, c->stream));
     return WL_OK;
 }
 
@@ -737,6 +971,10 @@ extern "C" int wl_set_option(wl_ctx *c, int option, int value) {
             return fail(WL_E_ARG, "workgroup size must be 64, 128 or 256");
         c->g.blk = value;
         return WL_OK;
+    case WL_OPT_RHS_BLOCK:
+        if (value < 0 || value > 4) return fail(WL_E_ARG, "rhs per lane must be 0 (automatic) .. 4");
+        c->rhs_block = value;
+        return WL_OK;
     default: return fail(WL_E_ARG, "unknown option %d", option);
     }
 }
@@ -746,6 +984,7 @@ extern "C" int wl_get_option(const wl_ctx *c, int option, int *value) {
     switch (option) {
     case WL_OPT_TRAVERSAL_CHUNK: *value = c->g.qc; return WL_OK;
     case WL_OPT_BLOCK: *value = c->g.blk; return WL_OK;
+    case WL_OPT_RHS_BLOCK: *value = c->rhs_block; return WL_OK;
     default: return fail(WL_E_ARG, "unknown option %d", option);
     }
 }
@@ -767,14 +1006,9 @@ extern "C" int wl_cg_solve_mixed(wl_ctx *c, wl_field *x, const wl_field *b, doub
     DeviceGuard dg(c->device);
     int rc = ensure_gauge(c, WL_F64);
     if (rc) return rc;
-    int worst = WL_OK;
-    for (int r = 0; r < b->nrhs; ++r) {
-        wl_cg_stats s{};
-        rc = cg_mixed(c, x->rhs_ptr(r), b->rhs_ptr(r), kappa, accuracy, maxiters, inner_prec, delta,
-                      update_every, &s);
-        if (rc && rc != WL_E_NOCONV) return rc;
-        if (rc) worst = rc;
-        if (stats) stats[r] = s;
-    }
-    return worst;
+    if (b->nrhs == 1)
+        return cg_mixed(c, x->data, b->data, kappa, accuracy, maxiters, inner_prec, delta,
+                        update_every, stats);
+    return cg_batched(c, x->data, b->data, b->nrhs, kappa, accuracy, maxiters, inner_prec, true,
+                      delta, update_every, stats);
 }

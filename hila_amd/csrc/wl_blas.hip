@@ -12,26 +12,30 @@ namespace wl {
 
 // ------------------------------------------------------------------- layout ------------
 
-// HILA AoS (V/2 sites x 24 doubles) -> component planes
+// HILA AoS (V/2 sites x 24 doubles) -> right-hand side j of an N-rhs field:
+// element (rhs j, component c, site k) at (j * 12 + c) * S + k.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_aos_to_soa(const double *__restrict__ aos,
-                                                       cx<R> *__restrict__ out, int vh, int S) {
+                                                       cx<R> *__restrict__ out, int vh, int S,
+                                                       int N, int j) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= vh) return;
     const double *src = aos + (size_t)24 * k;
 #pragma unroll
-    for (int c = 0; c < 12; ++c) out[c * S + k] = {(R)src[2 * c], (R)src[2 * c + 1]};
+    for (int c = 0; c < 12; ++c)
+        out[((size_t)j * 12 + c) * S + k] = {(R)src[2 * c], (R)src[2 * c + 1]};
 }
 
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_soa_to_aos(const cx<R> *__restrict__ in,
-                                                       double *__restrict__ aos, int vh, int S) {
+                                                       double *__restrict__ aos, int vh, int S,
+                                                       int N, int j) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= vh) return;
     double *dst = aos + (size_t)24 * k;
 #pragma unroll
     for (int c = 0; c < 12; ++c) {
-        const cx<R> v = in[c * S + k];
+        const cx<R> v = in[((size_t)j * 12 + c) * S + k];
         dst[2 * c] = (double)v.x;
         dst[2 * c + 1] = (double)v.y;
     }
@@ -65,13 +69,13 @@ __global__ __launch_bounds__(kBlock) void k_convert(const cx<double> *__restrict
 
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_norm2(const cx<R> *__restrict__ f, int vh, int S,
-                                                  RedArgs ra, int fin, const int *done) {
+                                                  int N, int j, RedArgs ra, const int *done) {
     if (done && *done) return;
     const int k = blockIdx.x * kBlock + threadIdx.x;
     double s = 0;
     if (k < vh) {
 #pragma unroll
-        for (int c = 0; c < 12; ++c) s += (double)norm2(f[c * S + k]);
+        for (int c = 0; c < 12; ++c) s += (double)norm2(f[((size_t)j * 12 + c) * S + k]);
     }
     reduce_partial(s, ra);
 }
@@ -147,13 +151,12 @@ __global__ __launch_bounds__(kBlock) void k_diff_norm2(const cx<double> *__restr
     reduce_partial(s, ra);
 }
 
+// whole-field copy of n complex numbers
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_copy(const cx<R> *__restrict__ src,
-                                                 cx<R> *__restrict__ dst, int vh, int S) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= vh) return;
-#pragma unroll
-    for (int c = 0; c < 12; ++c) dst[c * S + k] = src[c * S + k];
+                                                 cx<R> *__restrict__ dst, size_t n) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 // Sum the n partials in workgroup order and apply `fin` (skipped once the CG has finished).
@@ -174,27 +177,27 @@ hipError_t launch_finish(const RedArgs &ra, int nparts, int fin, const int *done
 
 static inline dim3 grid_sites(const Geom &g) { return dim3(nblocks(g)); }
 
-hipError_t launch_aos_to_soa(int prec, const Geom &g, const double *aos, void *planes,
-                             hipStream_t s) {
+hipError_t launch_aos_to_soa(int prec, const Geom &g, const double *aos, void *planes, int N,
+                             int j, hipStream_t s) {
     if (prec == WL_F64)
         hipLaunchKernelGGL(k_aos_to_soa<double>, grid_sites(g), dim3(kBlock), 0, s, aos,
-                           (cx<double> *)planes, g.vh, g.stride);
+                           (cx<double> *)planes, g.vh, g.stride, N, j);
     else if (prec == WL_F32)
         hipLaunchKernelGGL(k_aos_to_soa<float>, grid_sites(g), dim3(kBlock), 0, s, aos,
-                           (cx<float> *)planes, g.vh, g.stride);
+                           (cx<float> *)planes, g.vh, g.stride, N, j);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-hipError_t launch_soa_to_aos(int prec, const Geom &g, const void *planes, double *aos,
-                             hipStream_t s) {
+hipError_t launch_soa_to_aos(int prec, const Geom &g, const void *planes, double *aos, int N,
+                             int j, hipStream_t s) {
     if (prec == WL_F64)
         hipLaunchKernelGGL(k_soa_to_aos<double>, grid_sites(g), dim3(kBlock), 0, s,
-                           (const cx<double> *)planes, aos, g.vh, g.stride);
+                           (const cx<double> *)planes, aos, g.vh, g.stride, N, j);
     else if (prec == WL_F32)
         hipLaunchKernelGGL(k_soa_to_aos<float>, grid_sites(g), dim3(kBlock), 0, s,
-                           (const cx<float> *)planes, aos, g.vh, g.stride);
+                           (const cx<float> *)planes, aos, g.vh, g.stride, N, j);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -220,14 +223,14 @@ hipError_t launch_gauge_convert(const Geom &g, int prec, const void *gauge64, vo
     return hipGetLastError();
 }
 
-hipError_t launch_norm2(int prec, const Geom &g, const void *f, const RedArgs &ra, int fin,
-                        const int *done, hipStream_t s) {
+hipError_t launch_norm2(int prec, const Geom &g, const void *f, int N, int j, const RedArgs &ra,
+                        int fin, const int *done, hipStream_t s) {
     if (prec == WL_F64)
         hipLaunchKernelGGL(k_norm2<double>, grid_sites(g), dim3(kBlock), 0, s,
-                           (const cx<double> *)f, g.vh, g.stride, ra, fin, done);
+                           (const cx<double> *)f, g.vh, g.stride, N, j, ra, done);
     else if (prec == WL_F32)
         hipLaunchKernelGGL(k_norm2<float>, grid_sites(g), dim3(kBlock), 0, s,
-                           (const cx<float> *)f, g.vh, g.stride, ra, fin, done);
+                           (const cx<float> *)f, g.vh, g.stride, N, j, ra, done);
     else
         return hipErrorInvalidValue;
     hipError_t e = hipGetLastError();
@@ -280,13 +283,16 @@ hipError_t launch_diff_norm2(const Geom &g, const void *b, const void *v, const 
     return e != hipSuccess ? e : launch_finish(ra, nblocks(g), fin, nullptr, s);
 }
 
-hipError_t launch_copy(int prec, const Geom &g, const void *src, void *dst, hipStream_t s) {
+hipError_t launch_copy(int prec, const Geom &g, const void *src, void *dst, int nrhs,
+                       hipStream_t s) {
+    const size_t n = (size_t)nrhs * 12 * g.stride;
+    const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
     if (prec == WL_F64)
-        hipLaunchKernelGGL(k_copy<double>, grid_sites(g), dim3(kBlock), 0, s,
-                           (const cx<double> *)src, (cx<double> *)dst, g.vh, g.stride);
+        hipLaunchKernelGGL(k_copy<double>, grid, dim3(kBlock), 0, s, (const cx<double> *)src,
+                           (cx<double> *)dst, n);
     else if (prec == WL_F32)
-        hipLaunchKernelGGL(k_copy<float>, grid_sites(g), dim3(kBlock), 0, s,
-                           (const cx<float> *)src, (cx<float> *)dst, g.vh, g.stride);
+        hipLaunchKernelGGL(k_copy<float>, grid, dim3(kBlock), 0, s, (const cx<float> *)src,
+                           (cx<float> *)dst, n);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

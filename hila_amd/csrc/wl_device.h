@@ -297,6 +297,8 @@ enum Fin : int {
     FIN_TRUE = 6,      // true_rr = s
     FIN_RRNEW_MIXED = 7,  // rrnew = s; convergence / reliable-update request / beta
     FIN_UPDATE = 8,    // s = ||b - M^dag M x||^2 after a reliable update: converged or restart
+    FIN_RRNEW_MIXED2 = 9,  // batched: rrnew = s; convergence / update request (beta: sync kernel)
+    FIN_UPDATE_FIRST = 10, // batched start: FIN_UPDATE applied to every rhs
 };
 
 struct RedArgs {
@@ -312,6 +314,7 @@ __device__ __forceinline__ void fin_apply(int fin, double s, const RedArgs &ra) 
     case FIN_SOURCE:
         st->source_norm = s;
         st->target_rr = st->acc2 * s;
+        if (s == 0.0) st->done = 1;  // b = 0: x = 0 is the solution
         break;
     case FIN_RR: st->rr = s; break;
     case FIN_ALPHA:
@@ -343,6 +346,15 @@ __device__ __forceinline__ void fin_apply(int fin, double s, const RedArgs &ra) 
             st->rr = s;
             if (st->iter >= st->maxiters) st->done = 2;
         }
+        break;
+    case FIN_RRNEW_MIXED2:
+        st->rrnew = s;
+        st->iter += 1;
+        st->iter_since += 1;
+        if (s < st->target_rr)
+            st->done = 4;
+        else if (s < st->delta2 * st->rr_update || st->iter_since >= st->update_every)
+            st->done = 3;
         break;
     case FIN_UPDATE:
         st->true_rr = s;

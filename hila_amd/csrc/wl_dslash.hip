@@ -19,6 +19,7 @@
 // Epilogues fuse the even-odd operator and CG vector updates (wilson.h:272-289,
 // conjugate_gradient.h:77-94) into the same pass; see Epi in wl_internal.h.
 
+#include "wl_hop.h"
 #include "wl_internal.h"
 
 namespace wl {
@@ -40,70 +41,6 @@ struct DslashArgs {
     int fin;
 };
 
-// Registers holding one hop's operands: neighbour spinor and link.
-template <typename R>
-struct HopBuf {
-    cx<R> w[4][3];
-    cx<R> u[3][3];
-};
-
-template <typename R>
-__device__ __forceinline__ void hop_load(HopBuf<R> &b, const cx<R> *__restrict__ in,
-                                         const cx<R> *__restrict__ link, int S, int kn, int kl) {
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) b.w[s][c] = in[(s * 3 + c) * S + kn];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) b.u[i][j] = ldc<true>(link + (i * 3 + j) * S + kl);
-}
-
-// acc += expand(U (1 + s g_mu) w) for a forward hop, expand(U^dag (1 - s g_mu) w) backward;
-// sg = -1 for a t hop across the antiperiodic edge.
-template <int MU, int SIGN, bool FWD, typename R>
-__device__ __forceinline__ void hop_compute(cx<R> (&acc)[4][3], const HopBuf<R> &b, R sg) {
-    cx<R> h[2][3], v[2][3];
-    if constexpr (FWD) {
-        project<MU, SIGN>(b.w, h);
-        su3_mul(b.u, h, v);
-    } else {
-        project<MU, -SIGN>(b.w, h);
-        su3_adj_mul(b.u, h, v);
-    }
-    if constexpr (MU == 3) {
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) v[q][c] = sg * v[q][c];
-    }
-    if constexpr (FWD)
-        expand_add<MU, SIGN>(acc, v);
-    else
-        expand_add<MU, -SIGN>(acc, v);
-}
-
-// Hop H = 2 mu + (0 forward, 1 backward): issue hop H+1's loads, then compute hop H.
-template <typename R, int SIGN, int H>
-__device__ __forceinline__ void hop_chain(cx<R> (&acc)[4][3], HopBuf<R> (&buf)[2],
-                                          const cx<R> *__restrict__ in, const cx<R> *gout,
-                                          const cx<R> *gin, int S, int k, const SiteNb &n, R sf,
-                                          R sb) {
-    if constexpr (H < 8) {
-        constexpr int HN = H + 1;
-        if constexpr (HN < 8) {
-            constexpr int MN = HN / 2;
-            if constexpr (HN % 2 == 0)
-                hop_load(buf[HN & 1], in, gout + MN * 9 * S, S, n.fwd[MN], k);
-            else
-                hop_load(buf[HN & 1], in, gin + MN * 9 * S, S, n.bwd[MN], n.bwd[MN]);
-        }
-        hop_compute<H / 2, SIGN, (H % 2 == 0)>(acc, buf[H & 1], (H % 2 == 0) ? sf : sb);
-        hop_chain<R, SIGN, H + 1>(acc, buf, in, gout, gin, S, k, n, sf, sb);
-    }
-}
-
 template <typename R, int SIGN, int EPI>
 __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
     if (A.done && *A.done) return;
@@ -111,19 +48,8 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
     const int S = A.g.stride;
     double nrm = 0;
     if (k < A.g.vh) {
-        const SiteNb n = neighbours(A.g, k, A.out_par);
-        const cx<R> *gout = A.gauge + (size_t)A.out_par * 36 * S;
-        const cx<R> *gin = A.gauge + (size_t)(1 - A.out_par) * 36 * S;
-        const R sf = (A.g.antiperiodic && n.fwd_edge_t) ? R(-1) : R(1);
-        const R sb = (A.g.antiperiodic && n.bwd_edge_t) ? R(-1) : R(1);
         cx<R> acc[4][3];
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) acc[s][c] = {R(0), R(0)};
-        HopBuf<R> buf[2];
-        hop_load(buf[0], A.in, gout, S, n.fwd[0], k);
-        hop_chain<R, SIGN, 0>(acc, buf, A.in, gout, gin, S, k, n, sf, sb);
+        hop_sum<R, SIGN>(acc, A.g, k, A.out_par, A.in, A.gauge, 1);
 
         const R a = A.a;
         if constexpr (EPI == EPI_SET) {

@@ -7,6 +7,8 @@
 #include <cstdint>
 #include <mutex>
 #include <string>
+#include <map>
+#include <tuple>
 #include <unordered_set>
 
 #include "wl_device.h"
@@ -52,17 +54,18 @@ struct DslashLaunch {
 // Kernel launchers (wl_dslash.hip / wl_blas.hip).  All enqueue on `s` and return hipError_t.
 hipError_t launch_dslash(int prec, const Geom &g, const DslashLaunch &L, hipStream_t s);
 
-hipError_t launch_aos_to_soa(int prec, const Geom &g, const double *aos, void *planes,
-                             hipStream_t s);
-hipError_t launch_soa_to_aos(int prec, const Geom &g, const void *planes, double *aos,
-                             hipStream_t s);
+// host AoS block <-> right-hand side j of an N-rhs field
+hipError_t launch_aos_to_soa(int prec, const Geom &g, const double *aos, void *planes, int N,
+                             int j, hipStream_t s);
+hipError_t launch_soa_to_aos(int prec, const Geom &g, const void *planes, double *aos, int N,
+                             int j, hipStream_t s);
 hipError_t launch_gauge_to_soa(const Geom &g, int mu, const double *aos_full, void *gauge64,
                                hipStream_t s);
 hipError_t launch_gauge_convert(const Geom &g, int prec, const void *gauge64, void *gauge_lo,
                                 hipStream_t s);
-// sum over sites of ||f||^2 (FP64 accumulate), finished with `fin`
-hipError_t launch_norm2(int prec, const Geom &g, const void *f, const RedArgs &ra, int fin,
-                        const int *done, hipStream_t s);
+// sum over sites of ||f_j||^2 for rhs j of an N-rhs field (FP64 accumulate), then `fin`
+hipError_t launch_norm2(int prec, const Geom &g, const void *f, int N, int j, const RedArgs &ra,
+                        int fin, const int *done, hipStream_t s);
 // r = b - DDp; p = r; reduce ||r||^2 -> FIN_RR   (conjugate_gradient.h:67-72)
 hipError_t launch_cg_init(const Geom &g, const void *b, const void *DDp, void *r, void *p,
                           const RedArgs &ra, hipStream_t s);
@@ -75,8 +78,44 @@ hipError_t launch_residual_update(int prec, const Geom &g, const void *b, const 
 // reduce ||b - v||^2 with `fin`
 hipError_t launch_diff_norm2(const Geom &g, const void *b, const void *v, const RedArgs &ra,
                              int fin, hipStream_t s);
-// copy f64 even field v into the all-zero-if-null convention: y = v
-hipError_t launch_copy(int prec, const Geom &g, const void *src, void *dst, hipStream_t s);
+// dst = src for whole fields of nrhs right-hand sides
+hipError_t launch_copy(int prec, const Geom &g, const void *src, void *dst, int nrhs,
+                       hipStream_t s);
+
+// ---- multi-rhs (wl_multi.hip) ----
+// One launch of the register-blocked multi-rhs hopping kernel: right-hand sides [j0, j0+NB),
+// NR per lane.  Per-rhs partial sums (norm epilogues) go to partials[rhs * mr_segments + seg].
+struct MrLaunch {
+    void *out;
+    const void *in;
+    const void *xin;
+    const void *gauge;
+    double a;
+    int out_par, sign, epi;
+    int j0, NB, NR;
+    const int *all_done;      // may be null
+    CgState *st;              // per-rhs states (skip finished rhs, alpha); may be null
+    void *x;
+    const void *p;
+    void *r;
+    double *partials;
+};
+hipError_t launch_dslash_mr(int prec, const Geom &g, const MrLaunch &L, hipStream_t s);
+int mr_segments(const Geom &g);
+hipError_t launch_finish_mr(double *partials, int pstride, int n, int N, CgState *st,
+                            double *out, int fin, const int *all_done, hipStream_t s);
+// BLAS-1 kernels over N right-hand sides: per-rhs partials at partials[rhs * nblocks(g) + b]
+hipError_t launch_norm2_mr(int prec, const Geom &g, const void *f, int N, double *partials,
+                           hipStream_t s);
+hipError_t launch_cg_init_mr(const Geom &g, const void *b, const void *ddp, void *r, void *p,
+                             int N, double *partials, hipStream_t s);
+hipError_t launch_residual_update_mr(int prec, const Geom &g, const void *b, const void *ddp,
+                                     void *r, const CgState *st, int first, int N,
+                                     double *partials, hipStream_t s);
+hipError_t launch_xpay_mr(int prec, const Geom &g, void *p, const void *r, const CgState *st,
+                          const int *all_done, int N, hipStream_t s);
+hipError_t launch_batch_sync(CgState *st, int N, int mixed, int *flags, hipStream_t s);
+hipError_t launch_zero_rhs(const Geom &g, void *x, int j, hipStream_t s);
 
 // one-workgroup kernel: sum nparts partials in order, apply `fin` (skipped when *done != 0)
 hipError_t launch_finish(const RedArgs &ra, int nparts, int fin, const int *done, hipStream_t s);
@@ -96,9 +135,8 @@ struct wl_field {
     void *data;
     size_t rhs_elems;  // complex elements per rhs = 12 * stride
     size_t bytes;
-    void *rhs_ptr(int r) const {
-        return (char *)data + (size_t)r * rhs_elems * wl::prec_bytes(prec);
-    }
+    // Layout: element (rhs j, component c, site k) at (j * 12 + c) * stride + k: every rhs is
+    // a set of 12 component planes.
 };
 
 struct wl_ctx {
@@ -123,6 +161,15 @@ struct wl_ctx {
     wl_field *tmp_odd32 = nullptr;
     wl_field *cg_r = nullptr, *cg_p = nullptr, *cg_Dp = nullptr, *cg_DDp = nullptr;
     wl_field *mx_r = nullptr, *mx_p = nullptr, *mx_Dp = nullptr;  // inner-precision Krylov vectors
+    // multi-rhs work fields, keyed by (slot, precision, parity, nrhs)
+    std::map<std::tuple<int, int, int, int>, wl_field *> work;
+    size_t partials_cap = 0;           // doubles in `partials`
+    wl::CgState *st_mr = nullptr;      // per-rhs CG states (device) and pinned host mirror
+    wl::CgState *st_mr_host = nullptr;
+    int st_mr_cap = 0;
+    int *flags_d = nullptr;            // batched CG: [0] all finished, [1] update pending
+    int *flags_h = nullptr;
+    int rhs_block = 0;                 // rhs per lane of the multi-rhs kernel (0: automatic)
     // fields created through wl_field_create and not yet destroyed; wl_destroy orphans them
     std::unordered_set<wl_field *> user_fields;
     std::mutex mu;

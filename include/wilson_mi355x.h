@@ -104,7 +104,9 @@ int wl_gauge_upload(wl_ctx *ctx, const double *const U[4]);
 /* ---- device fields (one parity, nrhs right-hand sides) -------------------------------- */
 
 /* A device Wilson-vector field on the sites of one parity (WL_EVEN / WL_ODD), holding nrhs
- * independent vectors, stored in precision prec. */
+ * independent vectors (right-hand sides), stored in precision prec.  The operator applies to
+ * all right-hand sides of a field in one pass, each lane loading a link once for several of
+ * them (WL_OPT_RHS_BLOCK). */
 int wl_field_create(wl_ctx *ctx, int parity, int prec, int nrhs, wl_field **out);
 /* Fields should be destroyed before their context.  A field still alive when wl_destroy runs
  * loses its device memory and becomes an orphan: every call on it returns WL_E_STATE except
@@ -127,7 +129,7 @@ int wl_field_norm2(const wl_field *f, int rhs, double *out);
 
 /* Hopping half-operator, HILA's Dirac_Wilson_hop_set(gauge, kappa, in, out, par, sign)
  * (wilson.h:53-95): out = -kappa * H_s in, with H_s the sum over the 8 hops of
- * D_s above; out has parity opposite to in.  All nrhs vectors are processed. */
+ * D_s above; out has parity opposite to in.  All nrhs vectors are processed together. */
 int wl_dslash_eo(wl_ctx *ctx, wl_field *out, const wl_field *in, double kappa, int sign);
 
 /* out = M_s in on even sites: sign=+1 is Dirac_Wilson_evenodd::apply (wilson.h:272-279),
@@ -138,9 +140,10 @@ int wl_meo_apply(wl_ctx *ctx, wl_field *out, const wl_field *in, double kappa, i
 
 /* CG on M^dag M, HILA's CG<Dirac_Wilson_evenodd>(D, accuracy, maxiters).apply(b, x)
  * (conjugate_gradient.h:33-104): x is the initial guess on entry and the solution on exit;
- * stops when ||r||^2 < accuracy^2 * ||b||^2.  FP64 throughout.  Each of the nrhs vectors is
- * solved in turn; stats (may be NULL) points to nrhs entries.  Returns WL_E_NOCONV (with x
- * and stats filled) if maxiters is reached first for any rhs. */
+ * stops when ||r||^2 < accuracy^2 * ||b||^2.  FP64 throughout.  With nrhs > 1 the right-hand
+ * sides are solved together (batched CG): each keeps its own recurrence and stopping test and
+ * all share every operator application.  stats (may be NULL) points to nrhs entries.  Returns
+ * WL_E_NOCONV (with x and stats filled) if maxiters is reached first for any rhs. */
 int wl_cg_solve(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa, double accuracy,
                 int maxiters, wl_cg_stats *stats);
 
@@ -165,8 +168,10 @@ int wl_cg_solve_mixed(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa,
  *       visited: 0 = memory order; n > 0 = chunks of n segments of one time slice, visited for
  *       every t before the next chunk (n must divide the segments per slice); -1 = default.
  *   WL_OPT_BLOCK  workgroup size 64, 128 or 256; 0 = default for the volume.
- * Results do not depend on either option; only the speed does. */
-enum { WL_OPT_TRAVERSAL_CHUNK = 1, WL_OPT_BLOCK = 2 };
+ *   WL_OPT_RHS_BLOCK  right-hand sides per lane of the multi-rhs kernel (register blocking,
+ *       1..4; 0 = default: 4 in FP32, 2 in FP64).  A lane loads each link once for them.
+ * Results do not depend on these options; only the speed does. */
+enum { WL_OPT_TRAVERSAL_CHUNK = 1, WL_OPT_BLOCK = 2, WL_OPT_RHS_BLOCK = 3 };
 int wl_set_option(wl_ctx *ctx, int option, int value);
 int wl_get_option(const wl_ctx *ctx, int option, int *value);
 
