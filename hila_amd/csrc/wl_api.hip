@@ -206,8 +206,18 @@ int rhs_per_lane(const wl_ctx *c, int prec, int N) {
     return std::max(1, std::min(nr, std::min(N, cap)));
 }
 
-// out = a H_sign in (+ xin, fused CG update, ...) for all N right-hand sides in one launch;
-// per-rhs partial sums land in c->partials rows, mr_segments apart.
+// Right-hand sides per launch (a multiple of the per-lane count): each launch is one pass over
+// the lattice, so a smaller group keeps the neighbour-spinor footprint of a segment small
+// enough for the XCD L2, at one extra link read per pass.
+int rhs_per_launch(const wl_ctx *c, int nr, int N) {
+    int nl = c->rhs_launch > 0 ? c->rhs_launch : N;
+    nl = std::max(nr, nl / nr * nr);
+    return std::min(nl, N);
+}
+
+// out = a H_sign in (+ xin, fused CG update, ...) for all N right-hand sides, in passes of
+// rhs_per_launch right-hand sides; per-rhs partial sums land in c->partials rows,
+// mr_segments apart.
 int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, const void *xin,
                   double a, int sign, int epi, int N, const int *all_done, CgState *st, void *x,
                   const void *p, void *r, int *pstride_out) {
@@ -216,25 +226,29 @@ int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, c
         int rc = ensure_partials(c, (size_t)N * pstride);
         if (rc) return rc;
     }
-    MrLaunch L{};
-    L.out = out;
-    L.in = in;
-    L.xin = xin;
-    L.gauge = gauge_for(c, prec);
-    L.a = a;
-    L.out_par = out_par;
-    L.sign = sign;
-    L.epi = epi;
-    L.j0 = 0;
-    L.NB = N;
-    L.NR = rhs_per_lane(c, prec, N);
-    L.all_done = all_done;
-    L.st = st;
-    L.x = x;
-    L.p = p;
-    L.r = r;
-    L.partials = c->partials;
-    HIPCHK(launch_dslash_mr(prec, c->g, L, c->stream));
+    const int nr = rhs_per_lane(c, prec, N);
+    const int nl = rhs_per_launch(c, nr, N);
+    for (int j0 = 0; j0 < N; j0 += nl) {
+        MrLaunch L{};
+        L.out = out;
+        L.in = in;
+        L.xin = xin;
+        L.gauge = gauge_for(c, prec);
+        L.a = a;
+        L.out_par = out_par;
+        L.sign = sign;
+        L.epi = epi;
+        L.j0 = j0;
+        L.NB = std::min(nl, N - j0);
+        L.NR = nr;
+        L.all_done = all_done;
+        L.st = st;
+        L.x = x;
+        L.p = p;
+        L.r = r;
+        L.partials = c->partials;
+        HIPCHK(launch_dslash_mr(prec, c->g, L, c->stream));
+    }
     if (pstride_out) *pstride_out = pstride;
     return WL_OK;
 }
@@ -975,6 +989,10 @@ extern "C" int wl_set_option(wl_ctx *c, int option, int value) {
         if (value < 0 || value > 4) return fail(WL_E_ARG, "rhs per lane must be 0 (automatic) .. 4");
         c->rhs_block = value;
         return WL_OK;
+    case WL_OPT_RHS_LAUNCH:
+        if (value < 0 || value > 64) return fail(WL_E_ARG, "rhs per launch must be 0 (all) .. 64");
+        c->rhs_launch = value;
+        return WL_OK;
     default: return fail(WL_E_ARG, "unknown option %d", option);
     }
 }
@@ -985,6 +1003,7 @@ extern "C" int wl_get_option(const wl_ctx *c, int option, int *value) {
     case WL_OPT_TRAVERSAL_CHUNK: *value = c->g.qc; return WL_OK;
     case WL_OPT_BLOCK: *value = c->g.blk; return WL_OK;
     case WL_OPT_RHS_BLOCK: *value = c->rhs_block; return WL_OK;
+    case WL_OPT_RHS_LAUNCH: *value = c->rhs_launch; return WL_OK;
     default: return fail(WL_E_ARG, "unknown option %d", option);
     }
 }

@@ -170,6 +170,7 @@ struct wl_ctx {
     int *flags_d = nullptr;            // batched CG: [0] all finished, [1] update pending
     int *flags_h = nullptr;
     int rhs_block = 0;                 // rhs per lane of the multi-rhs kernel (0: automatic)
+    int rhs_launch = 0;                // rhs per launch (pass over the lattice; 0: automatic)
     // fields created through wl_field_create and not yet destroyed; wl_destroy orphans them
     std::unordered_set<wl_field *> user_fields;
     std::mutex mu;

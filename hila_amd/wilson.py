@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libhila_wilson.so")
 WL_OK, WL_E_ARG, WL_E_OOM, WL_E_DEVICE, WL_E_NOCONV, WL_E_STATE, WL_E_NOTIMPL = 0, -1, -2, -3, -4, -5, -6
 EVEN, ODD = 1, 2
 F64, F32, F16 = 0, 1, 2
-OPT_TRAVERSAL_CHUNK, OPT_BLOCK, OPT_RHS_BLOCK = 1, 2, 3
+OPT_TRAVERSAL_CHUNK, OPT_BLOCK, OPT_RHS_BLOCK, OPT_RHS_LAUNCH = 1, 2, 3, 4
 
 EXPORTED_SYMBOLS = [
     "wl_version", "wl_last_error", "wl_create", "wl_destroy", "wl_sync", "wl_volume",

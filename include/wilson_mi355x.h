@@ -170,8 +170,11 @@ int wl_cg_solve_mixed(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa,
  *   WL_OPT_BLOCK  workgroup size 64, 128 or 256; 0 = default for the volume.
  *   WL_OPT_RHS_BLOCK  right-hand sides per lane of the multi-rhs kernel (register blocking,
  *       1..4; 0 = default: 4 in FP32, 2 in FP64).  A lane loads each link once for them.
+ *   WL_OPT_RHS_LAUNCH  right-hand sides per launch, i.e. per pass over the lattice (rounded
+ *       to a multiple of the per-lane count; 0 = default).  Fewer per pass keep the neighbour
+ *       spinors of a pass in cache; each pass reads the links again.
  * Results do not depend on these options; only the speed does. */
-enum { WL_OPT_TRAVERSAL_CHUNK = 1, WL_OPT_BLOCK = 2, WL_OPT_RHS_BLOCK = 3 };
+enum { WL_OPT_TRAVERSAL_CHUNK = 1, WL_OPT_BLOCK = 2, WL_OPT_RHS_BLOCK = 3, WL_OPT_RHS_LAUNCH = 4 };
 int wl_set_option(wl_ctx *ctx, int option, int value);
 int wl_get_option(const wl_ctx *ctx, int option, int *value);
 
