@@ -18,6 +18,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -103,6 +104,277 @@ def mixed_precision_config(W, n):
     return out
 
 
+def even_point_seed(W, dims, seed):
+    """First seed >= `seed` whose point-source site is even (CG acts on even sites)."""
+    Vh = int(np.prod(dims)) // 2
+    while True:
+        _, site = W.gen_point(dims, seed, 0, 0)
+        if site < Vh:
+            return seed
+        seed += 1
+
+
+def multi_rhs_config(W, n_plain=4):
+    """Batched multi-rhs configuration: 32^4, links exp(i 0.3 H) (seed 5), kappa 0.125;
+    12 point sources at one seeded even site (one per spin-colour pair) and 12 Gaussian
+    sources, solved together by the batched CG (FP32 inner, FP64 reliable updates) to 1e-10
+    true residual, with the multi-rhs kernel autotuned for 24 rhs.  Compared per rhs with
+    plain FP64 single-rhs CG (HILA's algorithm) on n_plain of the same sources."""
+    dims, kappa, N = (32, 32, 32, 32), 0.125, 24
+    U = W.gen_gauge(dims, "smooth", 0.3, 5)
+    lat = W.Lattice(dims)
+    lat.upload_gauge(U)
+    del U
+    D = W.Dirac_Wilson_evenodd(kappa, lat)
+    b = W.Field(lat, W.EVEN, W.F64, nrhs=N)
+    pseed = even_point_seed(W, dims, 8)
+    for sc in range(12):
+        pt, site = W.gen_point(dims, pseed, sc // 3, sc % 3)
+        b.upload(pt, rhs=sc)
+    for j in range(12):
+        b.upload(W.gen_spinor(dims, 100 + j, W.EVEN), rhs=12 + j)
+    t0 = time.perf_counter()
+    tune = lat.autotune(W.F32, N)
+    tune["seconds"] = round(time.perf_counter() - t0, 2)
+    print(f"# multi-rhs autotune: {tune}", file=sys.stderr, flush=True)
+    x = W.Field(lat, W.EVEN, W.F64, nrhs=N)
+    cg = W.CG(D, 1e-10, 10000, precision="f32", delta=0.1, update_every=100)
+    W.CG(D, 1e-10, 8, precision="f32").apply(b, x)  # warm-up: work fields, code paths
+    x.zero()
+    lat.sync()
+    t = time.perf_counter()
+    st = cg.apply(b, x)
+    t_batch = time.perf_counter() - t
+    print(f"# multi-rhs batched CG: {N} rhs in {t_batch * 1e3:.0f} ms", file=sys.stderr, flush=True)
+    b1, x1 = W.Field(lat), W.Field(lat)
+    cg64 = W.CG(D, 1e-10, 10000, precision="f64")
+    picks = [0, 7, 12, 19][:n_plain]
+    t_plain, diffs, it64 = [], [], []
+    for j in picks:
+        b1.upload(b.download(j))
+        x1.zero()
+        lat.sync()
+        t = time.perf_counter()
+        s64 = cg64.apply(b1, x1)[0]
+        t_plain.append(time.perf_counter() - t)
+        it64.append(s64["iterations"])
+        diffs.append(rel_diff(x.download(j), x1.download()))
+    per_rhs_batch = t_batch / N
+    per_rhs_plain = float(np.mean(t_plain))
+    out = {
+        "lattice": list(dims), "kappa": kappa, "nrhs": N,
+        "sources": "12 point (one even site, all spin-colour pairs) + 12 Gaussian",
+        "autotune": tune,
+        "batched_ms": round(t_batch * 1e3, 1),
+        "rhs_solves_per_s": round(N / t_batch, 3),
+        "iterations_max": max(s["iterations"] for s in st),
+        "reliable_updates_max": max(s["reliable_updates"] for s in st),
+        "max_true_resid": max(s["true_resid"] for s in st),
+        "all_converged": all(s["converged"] for s in st),
+        "plain_f64_single_rhs_ms_per_solve": round(per_rhs_plain * 1e3, 2),
+        "plain_f64_iterations": it64,
+        "speedup_per_rhs_vs_plain_f64": round(per_rhs_plain / per_rhs_batch, 3),
+        "solution_rel_diff_vs_f64_max": max(diffs),
+    }
+    lat.close()
+    return out
+
+
+SERVE_VOLUMES = ((16, 16, 16, 16), (24, 24, 24, 48), (32, 32, 32, 32))
+
+
+class _SolutionPool:
+    """Page-locked solution buffers (12 rhs each) per lattice size, recycled between requests."""
+
+    def __init__(self, W):
+        self.W = W
+        self.free = {d: [] for d in SERVE_VOLUMES}
+        self.lock = threading.Lock()
+
+    def get(self, dims):
+        with self.lock:
+            if self.free[dims]:
+                return self.free[dims].pop()
+        return self.W.HostBuffer((12, int(np.prod(dims)) // 2, 24))
+
+    def put(self, dims, buf):
+        with self.lock:
+            self.free[dims].append(buf)
+
+
+def serving_pool(W, rng, per_volume):
+    """Per lattice size: `per_volume` gauge fields exp(i eps H) with eps ~ U[0.2, 0.5] and
+    their own seeds, and 12 Gaussian sources, all in page-locked host memory."""
+    gauges, sources = {}, {}
+    for dims in SERVE_VOLUMES:
+        V = int(np.prod(dims))
+        lst = []
+        for _ in range(per_volume):
+            eps, seed = float(rng.uniform(0.2, 0.5)), int(rng.integers(1, 1 << 30))
+            hb = W.HostBuffer((4, V, 18))
+            hb.array[:] = W.gen_gauge(dims, "smooth", eps, seed)
+            lst.append((eps, seed, hb))
+        sb = W.HostBuffer((12, V // 2, 24))
+        for j in range(12):
+            sb.array[j] = W.gen_spinor(dims, int(rng.integers(1, 1 << 30)), W.EVEN)[: V // 2]
+        gauges[dims], sources[dims] = lst, sb
+    return gauges, sources
+
+
+def draw_requests(rng, n, per_volume):
+    """Request parameters: volume uniform over the three sizes, kappa ~ U[0.120, 0.1245],
+    tolerance from {1e-8, 1e-10, 1e-12}, 1..12 right-hand sides, gauge field from the pool."""
+    out = []
+    for _ in range(n):
+        dims = SERVE_VOLUMES[int(rng.integers(len(SERVE_VOLUMES)))]
+        out.append({"dims": dims, "g": int(rng.integers(per_volume)),
+                    "kappa": float(rng.uniform(0.120, 0.1245)),
+                    "tol": [1e-8, 1e-10, 1e-12][int(rng.integers(3))],
+                    "nrhs": int(rng.integers(1, 13))})
+    return out
+
+
+def run_served(W, S, reqs, gauges, sources, pool, arrivals=None):
+    """Submit reqs (at the given arrival offsets in seconds, else all at once) from a client
+    thread, wait for each; returns (results, wall seconds from first submission to last
+    completion)."""
+    import queue
+    q = queue.Queue()
+    results = [None] * len(reqs)
+    t_start = time.perf_counter()
+
+    def submitter():
+        for i, r in enumerate(reqs):
+            if arrivals is not None:
+                dt = arrivals[i] - (time.perf_counter() - t_start)
+                if dt > 0:
+                    time.sleep(dt)
+            d = r["dims"]
+            sol = pool.get(d)
+            t = S.submit(d, gauges[d][r["g"]][2].array, sources[d].array[: r["nrhs"]],
+                         sol.array[: r["nrhs"]], r["kappa"], r["tol"], 10000, precision="f32",
+                         delta=0.1, update_every=100)
+            q.put((i, t, sol))
+        q.put(None)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    while True:
+        item = q.get()
+        if item is None:
+            break
+        i, t, sol = item
+        res, st = S.wait(t)
+        if res["status"] != 0:
+            raise RuntimeError(f"served request {i} failed: {res}")
+        results[i] = res
+        pool.put(reqs[i]["dims"], sol)
+    wall = time.perf_counter() - t_start
+    th.join()
+    return results, wall
+
+
+def plain_f64_baseline(W, reqs, gauges, sources):
+    """The same requests solved one right-hand side at a time by plain FP64 CG (HILA's
+    algorithm, wl_cg_solve with one rhs): gauge upload, source upload, solve, download per rhs,
+    contexts resident per lattice size.  Returns wall seconds."""
+    lats = {}
+    for d in SERVE_VOLUMES:
+        lat = W.Lattice(d)
+        lats[d] = (lat, W.Field(lat), W.Field(lat))
+    t0 = time.perf_counter()
+    for r in reqs:
+        d = r["dims"]
+        lat, b1, x1 = lats[d]
+        lat.upload_gauge(gauges[d][r["g"]][2].array)
+        D = W.Dirac_Wilson_evenodd(r["kappa"], lat)
+        cg = W.CG(D, r["tol"], 10000, precision="f64")
+        for j in range(r["nrhs"]):
+            b1.upload(sources[d].array[j])
+            x1.zero()
+            st = cg.apply(b1, x1)[0]
+            if not st["converged"]:
+                raise RuntimeError(f"plain FP64 CG did not converge: {st}")
+            x1.download()
+    wall = time.perf_counter() - t0
+    for lat, b1, x1 in lats.values():
+        b1.close()
+        x1.close()
+        lat.close()
+    return wall
+
+
+def serving_config(W, n_poisson=60, n_closed=24, rate=20.0, per_volume=3, seed=11):
+    """Serving configuration: Poisson arrivals at `rate`/s of CG requests (volume from
+    {16^4, 24^3x48, 32^4}, kappa ~ U[0.120, 0.1245], tolerance from {1e-8, 1e-10, 1e-12},
+    1..12 Gaussian rhs, own gauge field exp(i eps H), eps ~ U[0.2, 0.5]) on the request server
+    (mixed precision FP32 inner, FP64 reliable updates, batched over each request's rhs).
+    Reports p50/p99 latency, solves/s and true residuals; and, on a closed batch of
+    n_closed requests submitted at once, served rhs-solves/s against plain FP64 single-rhs CG
+    on the same requests."""
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    gauges, sources = serving_pool(W, rng, per_volume)
+    print(f"# serving pool {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    pool = _SolutionPool(W)
+    closed = draw_requests(rng, n_closed, per_volume)
+    poisson = draw_requests(rng, n_poisson, per_volume)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_poisson))
+    S = W.Server(max_active=16)
+    # warm-up: one request per lattice size (slots created, code paths loaded)
+    warm = [{"dims": d, "g": 0, "kappa": 0.12, "tol": 1e-8, "nrhs": 12} for d in SERVE_VOLUMES]
+    run_served(W, S, warm, gauges, sources, pool)
+    res_c, wall_c = run_served(W, S, closed, gauges, sources, pool)
+    rhs_c = sum(r["nrhs"] for r in closed)
+    print(f"# served closed batch: {n_closed} requests, {rhs_c} rhs in {wall_c:.2f} s",
+          file=sys.stderr, flush=True)
+    res_p, wall_p = run_served(W, S, poisson, gauges, sources, pool, arrivals=arrivals)
+    info = S.info()
+    S.close()
+    lat_ms = np.array([r["latency_ms"] for r in res_p])
+    q_ms = np.array([r["queue_ms"] for r in res_p])
+    rhs_p = sum(r["nrhs"] for r in poisson)
+    print(f"# served poisson: {n_poisson} requests in {wall_p:.2f} s, p50 "
+          f"{np.percentile(lat_ms, 50):.0f} ms p99 {np.percentile(lat_ms, 99):.0f} ms",
+          file=sys.stderr, flush=True)
+    wall_plain = plain_f64_baseline(W, closed, gauges, sources)
+    print(f"# plain FP64 single-rhs CG on the closed batch: {wall_plain:.2f} s", file=sys.stderr,
+          flush=True)
+    tres = [r["true_resid"] for r in res_p]
+    tol_ok = sum(1 for r, q in zip(res_p, poisson) if r["true_resid"] <= q["tol"])
+    for d in gauges:
+        for _, _, hb in gauges[d]:
+            hb.close()
+        sources[d].close()
+    return {
+        "arrival_rate_per_s": rate,
+        "requests": n_poisson,
+        "rhs": rhs_p,
+        "requests_per_s": round(n_poisson / wall_p, 3),
+        "rhs_solves_per_s": round(rhs_p / wall_p, 2),
+        "p50_ms": round(float(np.percentile(lat_ms, 50)), 1),
+        "p99_ms": round(float(np.percentile(lat_ms, 99)), 1),
+        "mean_queue_ms": round(float(q_ms.mean()), 1),
+        "max_true_resid": float(max(tres)),
+        "requests_meeting_tolerance": tol_ok,
+        "precision": "f32 inner, f64 reliable updates and true residual",
+        "max_active": info["max_active"],
+        "slots": info["slots"],
+        "hbm_in_use_gb": round(info["bytes_in_use"] / 1e9, 2),
+        "closed_batch": {
+            "requests": n_closed, "rhs": rhs_c,
+            "served_s": round(wall_c, 3),
+            "served_rhs_solves_per_s": round(rhs_c / wall_c, 2),
+            "plain_f64_single_rhs_s": round(wall_plain, 3),
+            "plain_f64_rhs_solves_per_s": round(rhs_c / wall_plain, 2),
+            "speedup_vs_plain_f64": round(wall_plain / wall_c, 3),
+        },
+        "gauge_pool": f"{per_volume} gauge fields per lattice size (each with its own eps and "
+                      f"seed), 12 Gaussian sources per size; a request draws one gauge field "
+                      f"and its first nrhs sources",
+    }
+
+
 def cpu_baseline(U, psi, seconds=15.0):
     """The oracle's HILA-structured hopping term (projection pass + per-direction
     accumulation, wilson.h:53-95) on the host cores, on a bounded number of applications."""
@@ -138,6 +410,10 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-big", action="store_true",
                     help="skip the 48^3x96 mixed-precision configuration")
+    ap.add_argument("--no-multi", action="store_true",
+                    help="skip the batched multi-rhs configuration")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving configuration")
+    ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: --steps hopping launches, then --steps "
                          "plain field copies (12 planes of V/2 complex doubles each way) as "
@@ -208,11 +484,15 @@ def main(argv=None):
         cg_mixed["speedup_vs_f64"] = round(cg_mixed["solves_per_s"] / cg_out["solves_per_s"], 3)
         cg_mixed["solution_rel_diff_vs_f64"] = rel_diff(xmx.download(), x64.download())
     del x64, xmx
-    big = None
+    big = multi = serve = None
+    D = fin = fodd = None
+    lat.close()
     if a.cg_solves > 0 and not a.no_big:
-        D = fin = fodd = None
-        lat.close()
         big = mixed_precision_config(W, max(2, a.cg_solves // 2))
+    if a.cg_solves > 0 and not a.no_multi:
+        multi = multi_rhs_config(W)
+    if a.cg_solves > 0 and not a.no_serve:
+        serve = serving_config(W, n_poisson=a.serve_requests)
 
     traffic = None
     if os.path.exists(PMC_FILE):
@@ -245,6 +525,8 @@ def main(argv=None):
         "cg": cg_out,
         "cg_mixed_f32": cg_mixed,
         "mixed_48x48x48x96": big,
+        "multi_rhs_32x32x32x32": multi,
+        "serving": serve,
         "cpu_baseline": None,
     }
     if not a.no_cpu:

@@ -7,37 +7,42 @@
 // state.
 
 #include "wl_internal.h"
+#include "wl_store.h"
 
 namespace wl {
 
 // ------------------------------------------------------------------- layout ------------
 
-// HILA AoS (V/2 sites x 24 doubles) -> right-hand side j of an N-rhs field:
-// element (rhs j, component c, site k) at (j * 12 + c) * S + k.
-template <typename R>
+// HILA AoS (V/2 sites x 24 doubles) -> right-hand side j of an N-rhs field in storage T:
+// component c of site k of rhs j at element (j * planes + c) * S + k.
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_aos_to_soa(const double *__restrict__ aos,
-                                                       cx<R> *__restrict__ out, int vh, int S,
-                                                       int N, int j) {
+                                                       typename St<T>::E *__restrict__ out, int vh,
+                                                       int S, int N, int j) {
+    using R = typename St<T>::R;
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= vh) return;
     const double *src = aos + (size_t)24 * k;
+    cx<R> v[4][3];
 #pragma unroll
-    for (int c = 0; c < 12; ++c)
-        out[((size_t)j * 12 + c) * S + k] = {(R)src[2 * c], (R)src[2 * c + 1]};
+    for (int c = 0; c < 12; ++c) v[c / 3][c % 3] = {(R)src[2 * c], (R)src[2 * c + 1]};
+    st_site<T>(out + (size_t)j * St<T>::planes * S, S, k, v);
 }
 
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_soa_to_aos(const cx<R> *__restrict__ in,
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_soa_to_aos(const typename St<T>::E *__restrict__ in,
                                                        double *__restrict__ aos, int vh, int S,
                                                        int N, int j) {
+    using R = typename St<T>::R;
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= vh) return;
     double *dst = aos + (size_t)24 * k;
+    cx<R> v[4][3];
+    ld_site<T>(in + (size_t)j * St<T>::planes * S, S, k, v);
 #pragma unroll
     for (int c = 0; c < 12; ++c) {
-        const cx<R> v = in[((size_t)j * 12 + c) * S + k];
-        dst[2 * c] = (double)v.x;
-        dst[2 * c + 1] = (double)v.y;
+        dst[2 * c] = (double)v[c / 3][c % 3].x;
+        dst[2 * c + 1] = (double)v[c / 3][c % 3].y;
     }
 }
 
@@ -65,17 +70,31 @@ __global__ __launch_bounds__(kBlock) void k_convert(const cx<double> *__restrict
     out[i] = {(R)v.x, (R)v.y};
 }
 
+// FP64 links -> FP16 planes (SU(3) entries are bounded by 1: no scale)
+__global__ __launch_bounds__(kBlock) void k_convert_h2(const cx<double> *__restrict__ in,
+                                                       h2 *__restrict__ out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const cx<double> v = in[i];
+    h2 q;
+    q.x = (_Float16)(float)v.x;
+    q.y = (_Float16)(float)v.y;
+    out[i] = q;
+}
+
 // ------------------------------------------------------------------ reductions ---------
 
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_norm2(const cx<R> *__restrict__ f, int vh, int S,
-                                                  int N, int j, RedArgs ra, const int *done) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_norm2(const typename St<T>::E *__restrict__ f, int vh,
+                                                  int S, int N, int j, RedArgs ra, const int *done) {
     if (done && *done) return;
     const int k = blockIdx.x * kBlock + threadIdx.x;
     double s = 0;
     if (k < vh) {
+        cx<typename St<T>::R> w[4][3];
+        ld_site<T>(f + (size_t)j * St<T>::planes * S, S, k, w);
 #pragma unroll
-        for (int c = 0; c < 12; ++c) s += (double)norm2(f[((size_t)j * 12 + c) * S + k]);
+        for (int c = 0; c < 12; ++c) s += (double)norm2(w[c / 3][c % 3]);
     }
     reduce_partial(s, ra);
 }
@@ -159,6 +178,12 @@ __global__ __launch_bounds__(kBlock) void k_copy(const cx<R> *__restrict__ src,
     if (i < n) dst[i] = src[i];
 }
 
+__global__ __launch_bounds__(kBlock) void k_copy_words(const unsigned *__restrict__ src,
+                                                       unsigned *__restrict__ dst, size_t n) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 // Sum the n partials in workgroup order and apply `fin` (skipped once the CG has finished).
 __global__ __launch_bounds__(kBlock) void k_finish(RedArgs ra, int n, int fin, const int *done) {
     if (done && *done) return;
@@ -185,6 +210,9 @@ hipError_t launch_aos_to_soa(int prec, const Geom &g, const double *aos, void *p
     else if (prec == WL_F32)
         hipLaunchKernelGGL(k_aos_to_soa<float>, grid_sites(g), dim3(kBlock), 0, s, aos,
                            (cx<float> *)planes, g.vh, g.stride, N, j);
+    else if (prec == WL_F16)
+        hipLaunchKernelGGL(k_aos_to_soa<f16s>, grid_sites(g), dim3(kBlock), 0, s, aos, (h2 *)planes,
+                           g.vh, g.stride, N, j);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -198,6 +226,9 @@ hipError_t launch_soa_to_aos(int prec, const Geom &g, const void *planes, double
     else if (prec == WL_F32)
         hipLaunchKernelGGL(k_soa_to_aos<float>, grid_sites(g), dim3(kBlock), 0, s,
                            (const cx<float> *)planes, aos, g.vh, g.stride, N, j);
+    else if (prec == WL_F16)
+        hipLaunchKernelGGL(k_soa_to_aos<f16s>, grid_sites(g), dim3(kBlock), 0, s, (const h2 *)planes,
+                           aos, g.vh, g.stride, N, j);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -218,6 +249,9 @@ hipError_t launch_gauge_convert(const Geom &g, int prec, const void *gauge64, vo
     if (prec == WL_F32)
         hipLaunchKernelGGL(k_convert<float>, grid, dim3(kBlock), 0, s,
                            (const cx<double> *)gauge64, (cx<float> *)gauge_lo, n);
+    else if (prec == WL_F16)
+        hipLaunchKernelGGL(k_convert_h2, grid, dim3(kBlock), 0, s, (const cx<double> *)gauge64,
+                           (h2 *)gauge_lo, n);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -231,6 +265,9 @@ hipError_t launch_norm2(int prec, const Geom &g, const void *f, int N, int j, co
     else if (prec == WL_F32)
         hipLaunchKernelGGL(k_norm2<float>, grid_sites(g), dim3(kBlock), 0, s,
                            (const cx<float> *)f, g.vh, g.stride, N, j, ra, done);
+    else if (prec == WL_F16)
+        hipLaunchKernelGGL(k_norm2<f16s>, grid_sites(g), dim3(kBlock), 0, s, (const h2 *)f, g.vh,
+                           g.stride, N, j, ra, done);
     else
         return hipErrorInvalidValue;
     hipError_t e = hipGetLastError();
@@ -285,7 +322,7 @@ hipError_t launch_diff_norm2(const Geom &g, const void *b, const void *v, const 
 
 hipError_t launch_copy(int prec, const Geom &g, const void *src, void *dst, int nrhs,
                        hipStream_t s) {
-    const size_t n = (size_t)nrhs * 12 * g.stride;
+    const size_t n = (size_t)nrhs * planes_of(prec) * g.stride;
     const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
     if (prec == WL_F64)
         hipLaunchKernelGGL(k_copy<double>, grid, dim3(kBlock), 0, s, (const cx<double> *)src,
@@ -293,6 +330,8 @@ hipError_t launch_copy(int prec, const Geom &g, const void *src, void *dst, int 
     else if (prec == WL_F32)
         hipLaunchKernelGGL(k_copy<float>, grid, dim3(kBlock), 0, s, (const cx<float> *)src,
                            (cx<float> *)dst, n);
+    else if (prec == WL_F16)  // 4-byte elements: the FP16 pairs and the FP32 scale plane
+        hipLaunchKernelGGL(k_copy_words, grid, dim3(kBlock), 0, s, (const unsigned *)src, (unsigned *)dst, n);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

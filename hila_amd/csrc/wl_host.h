@@ -1,0 +1,90 @@
+// Host-side helpers shared by the translation units of libhila_wilson.so that implement the C
+// ABI: error reporting, device selection, device-memory accounting, work fields and the
+// operator building blocks (one or many right-hand sides) that the solvers compose.
+#pragma once
+
+#include <cstdarg>
+
+#include "wl_internal.h"
+
+namespace wl {
+
+// Store a formatted message as the calling thread's last error and return `code`.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+const char *last_error();
+
+#define HIPCHK(expr)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return ::wl::fail(WL_E_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                              __FILE__, __LINE__);                                              \
+    } while (0)
+
+// Make a device current for the calling thread; restore the previous one on exit.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// Device allocations of a context, counted in wl_ctx::dev_bytes.
+int dev_alloc(wl_ctx *c, void **p, size_t bytes, const char *what);
+void dev_free(wl_ctx *c, void *p, size_t bytes);
+
+int ensure_staging(wl_ctx *c, size_t bytes);
+int field_new(wl_ctx *c, int parity, int prec, int nrhs, wl_field **out);
+void field_free(wl_field *f);
+int ensure_tmp(wl_ctx *c, int prec, wl_field **slot, int parity);
+RedArgs red_args(wl_ctx *c);
+const void *gauge_for(wl_ctx *c, int prec);
+int ensure_gauge(wl_ctx *c, int prec);
+int check_sign(int sign);
+int ensure_partials(wl_ctx *c, size_t n);
+int ensure_st_mr(wl_ctx *c, int N);
+
+// out_par = a * H_sign in (+ xin), one rhs
+int dslash_one(wl_ctx *c, int prec, void *out, int out_par, const void *in, const void *xin,
+               double a, int sign, int epi, const int *done, int fin);
+// M_s in -> out (one rhs) through the odd temporary tmp
+int meo_one(wl_ctx *c, int prec, void *out, const void *in, void *tmp, double kappa, int sign,
+            int epi2, const int *done, int fin2);
+
+// Work fields of the solvers, one per (slot, precision, parity), created on first use with at
+// least the requested number of right-hand sides and kept resident with the context.
+enum WorkSlot { W_R = 0, W_P, W_DP, W_T, W_DP64, W_DDP64, W_T64, W_X };
+int work_field(wl_ctx *c, int slot, int prec, int parity, int nrhs, wl_field **out);
+
+int rhs_per_lane(const wl_ctx *c, int prec, int N);
+// rhs per launch for a requested count: a multiple of nr (at least nr), at most N
+int rhs_per_launch_of(int want, int nr, int N);
+int rhs_per_launch(const wl_ctx *c, int nr, int N);
+// out = a H_sign in (+ epilogue) for N right-hand sides; per-rhs partial sums land in
+// c->partials rows, *pstride_out apart.
+int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, const void *xin,
+                  double a, int sign, int epi, int N, const int *all_done, CgState *st, void *x,
+                  const void *p, void *r, int *pstride_out);
+// M_s for N right-hand sides (even fields) through the odd work field t
+int meo_mr(wl_ctx *c, int prec, void *out, const void *in, void *t, double kappa, int sign,
+           int epi2, int N, const int *all_done, CgState *st, int *pstride_out);
+
+// The configuration wl_autotune chose for (lattice size, precision, nrhs) in this process, if
+// any: rhs per lane, rhs per launch, workgroup size.
+bool tuned_config(const Geom &g, int prec, int nrhs, int *nr, int *nl, int *blk);
+
+// Upload / download of right-hand side j of an N-rhs field from / to host AoS spinors
+// (V/2 x 24 doubles, EVEN_SITES_FIRST order within the parity), enqueued on the context's
+// stream through its staging buffer.  The host buffer must stay valid until the stream has
+// passed the copy.
+int field_upload_async(wl_ctx *c, void *data, int prec, int N, int j, const double *host);
+int field_download_async(wl_ctx *c, const void *data, int prec, int N, int j, double *host);
+// Gauge upload (U[mu]: V x 18 doubles), enqueued on the context's stream.
+int gauge_upload_async(wl_ctx *c, const double *const U[4]);
+
+}  // namespace wl

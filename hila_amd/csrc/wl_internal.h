@@ -93,6 +93,7 @@ struct MrLaunch {
     double a;
     int out_par, sign, epi;
     int j0, NB, NR;
+    bool link_nt;             // nontemporal link loads
     const int *all_done;      // may be null
     CgState *st;              // per-rhs states (skip finished rhs, alpha); may be null
     void *x;
@@ -111,10 +112,12 @@ hipError_t launch_cg_init_mr(const Geom &g, const void *b, const void *ddp, void
                              int N, double *partials, hipStream_t s);
 hipError_t launch_residual_update_mr(int prec, const Geom &g, const void *b, const void *ddp,
                                      void *r, const CgState *st, int first, int N,
-                                     double *partials, hipStream_t s);
+                                     double *partials, const int *gate, hipStream_t s);
 hipError_t launch_xpay_mr(int prec, const Geom &g, void *p, const void *r, const CgState *st,
                           const int *all_done, int N, hipStream_t s);
-hipError_t launch_batch_sync(CgState *st, int N, int mixed, int *flags, hipStream_t s);
+// flags[0] all finished, flags[1] update pending, flags[2] no update pending; gate: skip if set
+hipError_t launch_batch_sync(CgState *st, int N, int mixed, int *flags, const int *gate,
+                             hipStream_t s);
 hipError_t launch_zero_rhs(const Geom &g, void *x, int j, hipStream_t s);
 
 // one-workgroup kernel: sum nparts partials in order, apply `fin` (skipped when *done != 0)
@@ -133,10 +136,11 @@ struct wl_field {
     int prec;
     int nrhs;
     void *data;
-    size_t rhs_elems;  // complex elements per rhs = 12 * stride
+    size_t rhs_elems;  // stored elements per rhs: 12 * stride (13 * stride for FP16: scales)
     size_t bytes;
-    // Layout: element (rhs j, component c, site k) at (j * 12 + c) * stride + k: every rhs is
-    // a set of 12 component planes.
+    // Layout: element (rhs j, component c, site k) at (j * P + c) * stride + k: every rhs is
+    // a set of 12 component planes (P = 12), plus a plane of FP32 site scales for FP16
+    // fields (P = 13).
 };
 
 struct wl_ctx {
@@ -147,8 +151,10 @@ struct wl_ctx {
     int antiperiodic = 1;
     void *gauge64 = nullptr;
     void *gauge32 = nullptr;
+    void *gauge16 = nullptr;
     bool gauge_ok = false;
     bool gauge32_ok = false;
+    bool gauge16_ok = false;
     void *staging = nullptr;
     size_t staging_bytes = 0;
     double *partials = nullptr;
@@ -161,8 +167,9 @@ struct wl_ctx {
     wl_field *tmp_odd32 = nullptr;
     wl_field *cg_r = nullptr, *cg_p = nullptr, *cg_Dp = nullptr, *cg_DDp = nullptr;
     wl_field *mx_r = nullptr, *mx_p = nullptr, *mx_Dp = nullptr;  // inner-precision Krylov vectors
-    // multi-rhs work fields, keyed by (slot, precision, parity, nrhs)
-    std::map<std::tuple<int, int, int, int>, wl_field *> work;
+    // solver work fields, keyed by (slot, precision, parity); each holds at least the rhs
+    // count of its largest use so far
+    std::map<std::tuple<int, int, int>, wl_field *> work;
     size_t partials_cap = 0;           // doubles in `partials`
     wl::CgState *st_mr = nullptr;      // per-rhs CG states (device) and pinned host mirror
     wl::CgState *st_mr_host = nullptr;
@@ -171,7 +178,9 @@ struct wl_ctx {
     int *flags_h = nullptr;
     int rhs_block = 0;                 // rhs per lane of the multi-rhs kernel (0: automatic)
     int rhs_launch = 0;                // rhs per launch (pass over the lattice; 0: automatic)
+    int link_load = 0;                 // multi-rhs link loads: 0 automatic, 1 cached, 2 nontemporal
     // fields created through wl_field_create and not yet destroyed; wl_destroy orphans them
     std::unordered_set<wl_field *> user_fields;
+    size_t dev_bytes = 0;              // device memory held by the context and its fields
     std::mutex mu;
 };

@@ -1,0 +1,137 @@
+// Storage formats of Wilson-vector and gauge fields on the device, and the per-site load and
+// store of a spinor in each.
+//
+//   double  FP64 complex planes, FP64 arithmetic
+//   float   FP32 complex planes, FP32 arithmetic
+//   f16s    FP16 complex planes (re, im as two halves in 4 bytes) with one FP32 scale per site
+//           and rhs, FP32 arithmetic.  A site's 24 reals are stored as v / m, m = the largest
+//           |re|, |im| of the site, so each stored half lies in [-1, 1] and the relative error
+//           of a component is at most 2^-11 of the site's largest one.  Links are stored as
+//           plain FP16 planes (SU(3) entries are bounded by 1).
+// A right-hand side of an N-rhs field is a block of `planes` planes of `stride` elements: the
+// 12 spin-colour component planes, and for f16s one more plane holding the FP32 site scales.
+#pragma once
+
+#include "wl_device.h"
+
+namespace wl {
+
+struct f16s {};  // tag of the FP16-with-site-scale format
+
+struct alignas(4) h2 {
+    _Float16 x, y;
+};
+
+template <typename T>
+struct St;
+template <>
+struct St<double> {
+    using R = double;       // arithmetic
+    using E = cx<double>;   // stored element
+    static constexpr int planes = 12;
+    static constexpr bool scaled = false;
+};
+template <>
+struct St<float> {
+    using R = float;
+    using E = cx<float>;
+    static constexpr int planes = 12;
+    static constexpr bool scaled = false;
+};
+template <>
+struct St<f16s> {
+    using R = float;
+    using E = h2;
+    static constexpr int planes = 13;
+    static constexpr bool scaled = true;
+};
+
+// One stored complex element, widened to the arithmetic type.  NT: nontemporal load.
+template <bool NT>
+__device__ __forceinline__ cx<double> lde(const cx<double> *p) {
+    return ldc<NT>(p);
+}
+template <bool NT>
+__device__ __forceinline__ cx<float> lde(const cx<float> *p) {
+    return ldc<NT>(p);
+}
+template <bool NT>
+__device__ __forceinline__ cx<float> lde(const h2 *p) {
+    unsigned bits;
+    if constexpr (NT)
+        bits = __builtin_nontemporal_load((const unsigned *)p);
+    else
+        bits = *(const unsigned *)p;
+    const h2 v = __builtin_bit_cast(h2, bits);
+    return {(float)v.x, (float)v.y};
+}
+
+// FP32 scale plane of the rhs block starting at `base` (f16s only).
+__device__ __forceinline__ const float *scale_plane(const h2 *base, int S) {
+    return reinterpret_cast<const float *>(base + (size_t)12 * S);
+}
+__device__ __forceinline__ float *scale_plane(h2 *base, int S) {
+    return reinterpret_cast<float *>(base + (size_t)12 * S);
+}
+
+// Spinor of site k of the rhs block `base` (component c at base[c * S + k]), in the
+// arithmetic type, scale applied.
+template <typename T, bool NT = false>
+__device__ __forceinline__ void ld_site(const typename St<T>::E *base, int S, int k,
+                                        cx<typename St<T>::R> (&w)[4][3]) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) w[s][c] = lde<NT>(base + (s * 3 + c) * S + k);
+    if constexpr (St<T>::scaled) {
+        const float m = scale_plane(base, S)[k];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) w[s][c] = m * w[s][c];
+    }
+}
+
+// Store the spinor v at site k of the rhs block `base`.  For f16s, v is replaced by the values
+// as stored (dequantised), so that norms taken from it are those of the stored field.
+template <typename T, bool NT = false>
+__device__ __forceinline__ void st_site(typename St<T>::E *base, int S, int k,
+                                        cx<typename St<T>::R> (&v)[4][3]) {
+    if constexpr (!St<T>::scaled) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) stc<NT>(base + (s * 3 + c) * S + k, v[s][c]);
+    } else {
+        float m = 0.f;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) m = fmaxf(m, fmaxf(fabsf(v[s][c].x), fabsf(v[s][c].y)));
+        const float inv = m > 0.f ? 1.0f / m : 0.f;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                h2 q;
+                q.x = (_Float16)(v[s][c].x * inv);
+                q.y = (_Float16)(v[s][c].y * inv);
+                const unsigned bits = __builtin_bit_cast(unsigned, q);
+                if constexpr (NT)
+                    __builtin_nontemporal_store(bits, (unsigned *)(base + (s * 3 + c) * S + k));
+                else
+                    *(unsigned *)(base + (s * 3 + c) * S + k) = bits;
+                v[s][c] = {m * (float)q.x, m * (float)q.y};
+            }
+        if constexpr (NT)
+            __builtin_nontemporal_store(m, scale_plane(base, S) + k);
+        else
+            scale_plane(base, S)[k] = m;
+    }
+}
+
+// Host side: precision code -> planes per rhs and bytes per stored element.
+inline constexpr int planes_of(int prec) { return prec == 2 ? 13 : 12; }
+inline constexpr int elem_bytes(int prec) { return prec == 0 ? 16 : (prec == 1 ? 8 : 4); }
+
+}  // namespace wl

@@ -24,16 +24,19 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libhila_wilson.so")
 
 # status codes and enums of include/wilson_mi355x.h
-WL_OK, WL_E_ARG, WL_E_OOM, WL_E_DEVICE, WL_E_NOCONV, WL_E_STATE, WL_E_NOTIMPL = 0, -1, -2, -3, -4, -5, -6
+WL_OK, WL_E_ARG, WL_E_OOM, WL_E_DEVICE, WL_E_NOCONV, WL_E_STATE, WL_E_NOTIMPL, WL_E_BUSY = (
+    0, -1, -2, -3, -4, -5, -6, -7)
 EVEN, ODD = 1, 2
 F64, F32, F16 = 0, 1, 2
-OPT_TRAVERSAL_CHUNK, OPT_BLOCK, OPT_RHS_BLOCK, OPT_RHS_LAUNCH = 1, 2, 3, 4
+OPT_TRAVERSAL_CHUNK, OPT_BLOCK, OPT_RHS_BLOCK, OPT_RHS_LAUNCH, OPT_PLANE_PAD, OPT_LINK_LOAD = 1, 2, 3, 4, 5, 6
 
 EXPORTED_SYMBOLS = [
     "wl_version", "wl_last_error", "wl_create", "wl_destroy", "wl_sync", "wl_volume",
     "wl_gauge_upload", "wl_field_create", "wl_field_destroy", "wl_field_upload",
     "wl_field_download", "wl_field_zero", "wl_field_copy", "wl_field_norm2", "wl_dslash_eo", "wl_meo_apply",
     "wl_cg_solve", "wl_cg_solve_mixed", "wl_set_option", "wl_get_option", "wl_time_op", "wl_gen_gauge", "wl_gen_spinor", "wl_gen_point",
+    "wl_server_create", "wl_server_submit", "wl_server_wait", "wl_server_info_get", "wl_server_destroy",
+    "wl_host_alloc", "wl_host_free", "wl_autotune",
 ]
 
 
@@ -50,6 +53,41 @@ class CGStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
+
+
+class SolveRequest(C.Structure):
+    _fields_ = [("dims", C.c_int * 4), ("antiperiodic_t", C.c_int), ("nrhs", C.c_int),
+                ("gauge", C.c_void_p * 4), ("source", C.c_void_p), ("solution", C.c_void_p),
+                ("kappa", C.c_double), ("tolerance", C.c_double), ("maxiters", C.c_int),
+                ("inner_prec", C.c_int), ("delta", C.c_double), ("update_every", C.c_int)]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("nrhs", C.c_int), ("iterations", C.c_int),
+                ("reliable_updates", C.c_int), ("true_resid", C.c_double), ("queue_ms", C.c_double),
+                ("latency_ms", C.c_double), ("message", C.c_char * 160)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["message"] = self.message.decode(errors="replace")
+        return d
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("rhs_block", C.c_int), ("rhs_launch", C.c_int), ("block", C.c_int),
+                ("candidates", C.c_int), ("us_per_rhs", C.c_double), ("us_per_rhs_default", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("budget_bytes", C.c_size_t), ("bytes_in_use", C.c_size_t), ("slots", C.c_int),
+                ("active", C.c_int), ("queued", C.c_int), ("max_active", C.c_int),
+                ("completed", C.c_int64), ("slots_created", C.c_int64), ("slots_released", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None
@@ -89,6 +127,14 @@ def lib():
         L.wl_gen_gauge.argtypes = [ip4, i, d, C.c_uint64, C.POINTER(vp)]
         L.wl_gen_spinor.argtypes = [ip4, C.c_uint64, i, vp]
         L.wl_gen_point.argtypes = [ip4, C.c_uint64, i, i, vp, C.POINTER(i64)]
+        L.wl_server_create.argtypes = [i, C.c_size_t, i, C.POINTER(vp)]
+        L.wl_server_submit.argtypes = [vp, C.POINTER(SolveRequest), C.POINTER(C.c_uint64)]
+        L.wl_server_wait.argtypes = [vp, C.c_uint64, i, C.POINTER(SolveResult), C.POINTER(CGStats)]
+        L.wl_server_info_get.argtypes = [vp, C.POINTER(ServerInfo)]
+        L.wl_server_destroy.argtypes = [vp]
+        L.wl_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
+        L.wl_host_free.argtypes = [vp]
+        L.wl_autotune.argtypes = [vp, i, i, C.POINTER(TuneResult)]
         for name in EXPORTED_SYMBOLS:
             if name not in ("wl_version", "wl_last_error", "wl_volume"):
                 getattr(L, name).restype = C.c_int
@@ -181,6 +227,13 @@ class Lattice:
         v = C.c_int(0)
         check(lib().wl_get_option(self.handle, int(option), C.byref(v)))
         return v.value
+
+    def autotune(self, prec, nrhs):
+        """Time the multi-rhs kernel's configurations for nrhs right-hand sides and keep the
+        fastest (wl_autotune); returns the choice."""
+        r = TuneResult()
+        check(lib().wl_autotune(self.handle, int(prec), int(nrhs), C.byref(r)))
+        return r.as_dict()
 
     def close(self):
         if self._h is not None:
@@ -307,7 +360,8 @@ class CG:
     precision="f64" runs HILA's algorithm in FP64.  precision="f32" runs the mixed-precision
     solver with reliable updates (inner operator and Krylov vectors in FP32, solution and
     reductions in FP64, FP64 true residual recomputed at every update and at the end);
-    delta and update_every set when reliable updates happen."""
+    precision="f16" the same with the inner fields stored in FP16 with a per-site scale (FP32
+    arithmetic); delta and update_every set when reliable updates happen."""
 
     def __init__(self, op: Dirac_Wilson_evenodd, accuracy=1e-12, maxiters=10000,
                  precision="f64", delta=1e-3, update_every=100):
@@ -326,9 +380,115 @@ class CG:
             rc = lib().wl_cg_solve(self.M.lat.handle, fout.handle, fin.handle, self.M.kappa,
                                    self.accuracy, self.maxiters, stats)
         else:
-            inner = {"f32": F32, "f64-mixed": F64}[self.precision]
+            inner = {"f32": F32, "f16": F16, "f64-mixed": F64}[self.precision]
             rc = lib().wl_cg_solve_mixed(self.M.lat.handle, fout.handle, fin.handle,
                                          self.M.kappa, self.accuracy, self.maxiters, inner,
                                          self.delta, self.update_every, stats)
         check(rc, allow=() if raise_on_noconv else (WL_E_NOCONV,))
         return [s.as_dict() for s in stats]
+
+
+# ------------------------------------------------------------------------ serving ------
+
+class HostBuffer:
+    """Page-locked host memory (wl_host_alloc) viewed as a float64 numpy array; request
+    buffers in it are copied by DMA without staging."""
+
+    def __init__(self, shape):
+        self.shape = tuple(int(x) for x in shape)
+        n = int(np.prod(self.shape))
+        p = C.c_void_p()
+        check(lib().wl_host_alloc(n * 8, C.byref(p)))
+        self._p = p
+        self.array = np.ctypeslib.as_array((C.c_double * n).from_address(p.value)).reshape(self.shape)
+
+    def close(self):
+        if self._p is not None and self._p.value:
+            self.array = None
+            lib().wl_host_free(self._p)
+        self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request server: CG solves of independent requests (own lattice size, gauge field,
+    kappa, tolerance and 1..64 right-hand sides) admitted by HBM budget and run concurrently
+    on one GPU (wl_server_*).  The arrays of a request must stay alive and unmodified until
+    wait() returns for it; the server keeps references to them until then."""
+
+    def __init__(self, device=0, budget_bytes=0, max_active=0):
+        h = C.c_void_p()
+        check(lib().wl_server_create(int(device), int(budget_bytes), int(max_active), C.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, dims, gauge, source, solution, kappa, tolerance, maxiters=10000,
+               precision="f32", delta=0.1, update_every=100, antiperiodic_t=True):
+        """gauge: (4, V, 18) float64; source, solution: (nrhs, V/2, 24) float64 (even sites).
+        precision 'f64' (FP64 CG), 'f32' or 'f16' (mixed, reliable updates).  Returns a ticket."""
+        dims = tuple(int(x) for x in dims)
+        V = int(np.prod(dims))
+        if gauge.shape != (4, V, 18) or gauge.dtype != np.float64 or not gauge.flags.c_contiguous:
+            raise ValueError("gauge must be a C-contiguous float64 array (4, V, 18)")
+        for a in (source, solution):
+            if a.ndim != 3 or a.shape[1:] != (V // 2, 24) or a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError("source/solution must be C-contiguous float64 arrays (nrhs, V/2, 24)")
+        if source.shape[0] != solution.shape[0]:
+            raise ValueError("source and solution differ in nrhs")
+        r = SolveRequest()
+        r.dims = (C.c_int * 4)(*dims)
+        r.antiperiodic_t = 1 if antiperiodic_t else 0
+        r.nrhs = int(source.shape[0])
+        for m in range(4):
+            r.gauge[m] = gauge[m].ctypes.data
+        r.source = source.ctypes.data
+        r.solution = solution.ctypes.data
+        r.kappa = float(kappa)
+        r.tolerance = float(tolerance)
+        r.maxiters = int(maxiters)
+        r.inner_prec = {"f64": F64, "f32": F32, "f16": F16}[precision]
+        r.delta = float(delta)
+        r.update_every = int(update_every)
+        t = C.c_uint64(0)
+        check(lib().wl_server_submit(self._h, C.byref(r), C.byref(t)))
+        self._keep[t.value] = (gauge, source, solution)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        """(result dict, per-rhs stats dicts) of a finished request; raises WilsonError with
+        WL_E_BUSY if it does not finish within timeout_ms."""
+        res = SolveResult()
+        arrays = self._keep.get(ticket)
+        nrhs = arrays[1].shape[0] if arrays is not None else 64
+        stats = (CGStats * nrhs)()
+        check(lib().wl_server_wait(self._h, int(ticket), int(timeout_ms), C.byref(res), stats))
+        self._keep.pop(ticket, None)
+        return res.as_dict(), [s.as_dict() for s in stats[: res.nrhs]]
+
+    def info(self):
+        i = ServerInfo()
+        check(lib().wl_server_info_get(self._h, C.byref(i)))
+        return i.as_dict()
+
+    def close(self):
+        if self._h is not None:
+            lib().wl_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -54,7 +54,8 @@ enum {
     WL_E_DEVICE = -3,  /* HIP runtime error                                                  */
     WL_E_NOCONV = -4,  /* CG reached maxiters without meeting the tolerance                  */
     WL_E_STATE = -5,   /* call out of order (e.g. no gauge field uploaded)                   */
-    WL_E_NOTIMPL = -6  /* option not provided by this build                                  */
+    WL_E_NOTIMPL = -6, /* option not provided by this build                                  */
+    WL_E_BUSY = -7     /* request not finished within the wait's time limit                  */
 };
 
 /* Parity, numbered as HILA's enum class Parity {none, even, odd, all}
@@ -173,10 +174,41 @@ int wl_cg_solve_mixed(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa,
  *   WL_OPT_RHS_LAUNCH  right-hand sides per launch, i.e. per pass over the lattice (rounded
  *       to a multiple of the per-lane count; 0 = default).  Fewer per pass keep the neighbour
  *       spinors of a pass in cache; each pass reads the links again.
+ *   WL_OPT_PLANE_PAD  sites of padding added to every component plane of the device layout
+ *       (a multiple of 64, 0..65536; -1 = default), so that planes do not sit at large
+ *       power-of-two distances and alias in the caches.  Only before the gauge field or any
+ *       field of the context exists (else WL_E_STATE).
+ *   WL_OPT_LINK_LOAD  link loads of the multi-rhs kernel: 0 = automatic (nontemporal when a
+ *       launch reads each link once), 1 = cached, 2 = nontemporal.
  * Results do not depend on these options; only the speed does. */
-enum { WL_OPT_TRAVERSAL_CHUNK = 1, WL_OPT_BLOCK = 2, WL_OPT_RHS_BLOCK = 3, WL_OPT_RHS_LAUNCH = 4 };
+enum {
+    WL_OPT_TRAVERSAL_CHUNK = 1,
+    WL_OPT_BLOCK = 2,
+    WL_OPT_RHS_BLOCK = 3,
+    WL_OPT_RHS_LAUNCH = 4,
+    WL_OPT_PLANE_PAD = 5,
+    WL_OPT_LINK_LOAD = 6
+};
 int wl_set_option(wl_ctx *ctx, int option, int value);
 int wl_get_option(const wl_ctx *ctx, int option, int *value);
+
+/* Autotuner of the multi-rhs hopping kernel (no counterpart in HILA).  For nrhs right-hand
+ * sides at precision prec it times the hopping term on scratch fields of this context for
+ * every candidate of rhs per lane (1, 2, 4 in FP32; 1, 2 in FP64), rhs per launch (1, 2, 4, 6,
+ * 12, ..., nrhs: the RHS block of one pass over the lattice) and workgroup size (64, 128, 256),
+ * and sets WL_OPT_RHS_BLOCK, WL_OPT_RHS_LAUNCH and WL_OPT_BLOCK to the fastest.  The choice is
+ * remembered per (lattice size, precision, nrhs) for the life of the process, so that later
+ * calls, and other contexts of the same size, take the same configuration without timing
+ * again (results then do not depend on timing noise).  The gauge field must be uploaded. */
+typedef struct wl_tune_result {
+    int rhs_block;
+    int rhs_launch;
+    int block;
+    int candidates;           /* configurations timed (0 when the choice was remembered)     */
+    double us_per_rhs;        /* hopping term of the chosen configuration, per rhs           */
+    double us_per_rhs_default;/* ... and of the default configuration                        */
+} wl_tune_result;
+int wl_autotune(wl_ctx *ctx, int prec, int nrhs, wl_tune_result *out);
 
 /* ---- diagnostics ---------------------------------------------------------------------- */
 
@@ -186,6 +218,81 @@ int wl_get_option(const wl_ctx *ctx, int option, int *value);
  * (its benchmark times the host loop, applications/benchmarks/bench_fermion.cpp:100-113). */
 int wl_time_op(wl_ctx *ctx, int what, wl_field *out, const wl_field *in, double kappa, int sign,
                int warmup, int iters, double *ms_per_launch);
+
+/* ---- request server: continuous batching of CG solves --------------------------------- */
+
+/* A server on one device runs CG solves of independent requests (each with its own lattice
+ * size, gauge field, kappa, tolerance and 1..64 right-hand sides) concurrently.  No
+ * counterpart in HILA, whose CG<Op>::apply is a blocking call on one operator
+ * (conjugate_gradient.h:46-104); each request is solved as CG<Dirac_Wilson_evenodd> would on
+ * its own (same operator, same stopping rule per rhs), batched over its right-hand sides.
+ *
+ * Admission: a request is admitted when a solve slot is free (at most max_active run at
+ * once) and the device memory of a slot for its lattice fits in the HBM budget.  Slots (a
+ * context with the gauge field, the rhs/solution fields and the CG work fields of a lattice
+ * size) stay resident after a request and are reused by the next request of that size; idle
+ * slots of other sizes are released when the budget requires it.  Requests are admitted in
+ * submission order.  The solves of admitted requests overlap on the device: each slot has its
+ * own stream, and the host drives all of them without waiting inside a solve. */
+typedef struct wl_server wl_server;
+
+typedef struct wl_solve_request {
+    int dims[4];
+    int antiperiodic_t;
+    int nrhs;                    /* 1..64                                                    */
+    const double *gauge[4];      /* U[mu]: V x 18 doubles, HILA layout (see top of file)      */
+    const double *source;        /* nrhs consecutive blocks of V/2 x 24 doubles: the even-site
+                                    half of HILA's AoS field (EVEN_SITES_FIRST)               */
+    double *solution;            /* nrhs blocks as `source`: the solutions (initial guess 0)  */
+    double kappa;
+    double tolerance;            /* CG accuracy: ||r||^2 < tolerance^2 ||b||^2 per rhs       */
+    int maxiters;
+    int inner_prec;              /* WL_F64: FP64 CG; WL_F32: mixed precision, reliable updates */
+    double delta;                /* mixed: update when ||r|| drops by delta (0 < delta < 1)   */
+    int update_every;            /* mixed: ... or after this many iterations                  */
+} wl_solve_request;
+
+typedef struct wl_solve_result {
+    int status;                  /* WL_OK, WL_E_NOCONV, or the error that ended the request  */
+    int nrhs;
+    int iterations;              /* largest over the rhs                                     */
+    int reliable_updates;        /* largest over the rhs                                     */
+    double true_resid;           /* largest FP64 ||b - M^dag M x|| / ||b|| over the rhs       */
+    double queue_ms;             /* submission -> admitted (slot assigned, inputs enqueued)  */
+    double latency_ms;           /* submission -> solutions written to host memory           */
+    char message[160];           /* error text when status is an error                        */
+} wl_solve_result;
+
+typedef struct wl_server_info {
+    size_t budget_bytes;
+    size_t bytes_in_use;         /* device memory of all slots                               */
+    int slots;                   /* resident slots                                           */
+    int active;                  /* requests being loaded, solved or written back            */
+    int queued;                  /* submitted, not yet admitted                              */
+    int max_active;
+    int64_t completed;
+    int64_t slots_created;
+    int64_t slots_released;
+} wl_server_info;
+
+/* hbm_budget_bytes = 0: half of the device memory free at creation.  max_active <= 0: 16. */
+int wl_server_create(int device, size_t hbm_budget_bytes, int max_active, wl_server **out);
+/* Queue a request.  The descriptor is copied; the host buffers it points to (gauge, source,
+ * solution) must stay valid and unmodified until the request completes.  Buffers from
+ * wl_host_alloc are copied by DMA without staging. */
+int wl_server_submit(wl_server *srv, const wl_solve_request *req, uint64_t *ticket);
+/* Wait up to timeout_ms (< 0: no limit, 0: poll) for a request.  On completion fills *res and,
+ * if per_rhs is not null, its nrhs entries, releases the ticket and returns WL_OK (the
+ * request's own outcome is res->status); WL_E_BUSY if not finished in time. */
+int wl_server_wait(wl_server *srv, uint64_t ticket, int timeout_ms, wl_solve_result *res,
+                   wl_cg_stats *per_rhs);
+int wl_server_info_get(wl_server *srv, wl_server_info *info);
+/* Finish every submitted request, then release all device memory and threads. */
+int wl_server_destroy(wl_server *srv);
+
+/* Page-locked host memory for request buffers (hipHostMalloc). */
+int wl_host_alloc(size_t bytes, void **out);
+int wl_host_free(void *p);
 
 /* ---- seeded synthetic inputs (host side, counter-based, independent of thread count) --- */
 

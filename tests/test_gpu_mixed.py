@@ -97,3 +97,66 @@ def test_mixed_cg_update_every_and_maxiters():
     x.zero()
     s = W.CG(D, 1e-12, maxiters=4, precision="f32").apply(b, x)[0]
     assert s["converged"] == 0 and s["iterations"] <= 4 + 8
+
+
+# FP16 storage with a per-site scale: each stored component is within 2^-11 of the site's
+# largest one; one application (input, links and output each rounded) stays within this
+# relative L2 error of the FP64 oracle.
+F16_TOL = 2e-3
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (8, 8, 8, 8), (4, 6, 8, 10)],
+                         ids=lambda d: "x".join(map(str, d)))
+def test_f16_operator_against_f64_oracle(dims):
+    lat, U = _setup(dims, "smooth", 0.3, 5)
+    orc = Oracle(dims)
+    D = W.Dirac_Wilson_evenodd(0.125, lat)
+    psi = W.gen_spinor(dims, 6, EVEN)
+    fin = W.Field(lat, EVEN, W.F16).upload(psi)
+    fo = W.Field(lat, ODD, W.F16)
+    fe = W.Field(lat, EVEN, W.F16)
+    for sign in (1, -1):
+        D.hop_set(fin, fo, sign)
+        err = rel(fo.to_full(), orc.hop_set(U, 0.125, psi, ODD, sign))
+        assert 1e-6 < err <= F16_TOL, err  # rounded to FP16, and within its tolerance
+        (D.apply if sign == 1 else D.dagger)(fin, fe)
+        assert rel(fe.to_full(), orc.meo(U, 0.125, psi, sign)) <= F16_TOL
+
+
+def test_f16_storage_round_trip_and_norm():
+    dims = (8, 8, 8, 8)
+    lat, _ = _setup(dims, "unit", 0.0, 1)
+    psi = W.gen_spinor(dims, 9, EVEN)[: lat.Vh]
+    psi[5] = 0.0                      # a zero site: scale 0, stored exactly
+    psi[7] *= 1e-30                   # a tiny site: the scale keeps its relative precision
+    f = W.Field(lat, EVEN, W.F16).upload(psi)
+    back = f.download()
+    site_max = np.abs(psi).max(axis=1, keepdims=True)
+    assert np.all(np.abs(back - psi) <= 2.0 ** -11 * site_max * (1 + 1e-6))
+    assert not back[5].any()
+    assert np.abs(back[7] - psi[7]).max() <= 2.0 ** -11 * np.abs(psi[7]).max()
+    assert abs(f.norm2() - float((back ** 2).sum())) <= 1e-6 * float((back ** 2).sum())
+    g = W.Field(lat, EVEN, W.F16).copy_from(f)
+    assert np.array_equal(g.download(), back)
+
+
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_f16_mixed_cg_reaches_true_residual(nrhs):
+    dims = (8, 8, 8, 8)
+    lat, U = _setup(dims, "smooth", 0.3, 5)
+    D = W.Dirac_Wilson_evenodd(0.124, lat)
+    b = W.Field(lat, EVEN, W.F64, nrhs=nrhs)
+    srcs = [W.gen_spinor(dims, 40 + j, EVEN) for j in range(nrhs)]
+    for j, sj in enumerate(srcs):
+        b.upload(sj, rhs=j)
+    x16, x64 = W.Field(lat, EVEN, W.F64, nrhs=nrhs), W.Field(lat, EVEN, W.F64, nrhs=nrhs)
+    st16 = W.CG(D, 1e-10, 10000, precision="f16", delta=0.1).apply(b, x16)
+    st64 = W.CG(D, 1e-10, 10000, precision="f64").apply(b, x64)
+    orc = Oracle(dims)
+    for j in range(nrhs):
+        assert st16[j]["converged"] and st16[j]["true_resid"] <= 1e-10
+        assert st16[j]["reliable_updates"] >= 1
+        x = x16.to_full(j)
+        r = orc.meo(U, 0.124, orc.meo(U, 0.124, x, 1), -1) - srcs[j]
+        assert np.linalg.norm(r) / np.linalg.norm(srcs[j]) <= 1.05e-10
+        assert rel(x16.download(j), x64.download(j)) <= 1e-8
