@@ -663,6 +663,10 @@ extern "C" int wl_set_option(wl_ctx *c, int option, int value) {
         if (value < 0 || value > 2) return fail(WL_E_ARG, "link load mode must be 0, 1 or 2");
         c->link_load = value;
         return WL_OK;
+    case WL_OPT_LDS_RESERVE:
+        if (value < 0 || value > 65536) return fail(WL_E_ARG, "LDS reserve must be 0..65536 bytes");
+        c->g.lds = value;
+        return WL_OK;
     case WL_OPT_PLANE_PAD: {
         if (value != -1 && (value < 0 || value > 65536 || value % 64 != 0))
             return fail(WL_E_ARG, "plane padding must be -1 or a multiple of 64 in 0..65536");
@@ -686,6 +690,7 @@ extern "C" int wl_get_option(const wl_ctx *c, int option, int *value) {
     case WL_OPT_RHS_LAUNCH: *value = c->rhs_launch; return WL_OK;
     case WL_OPT_PLANE_PAD: *value = c->g.stride - plane_stride(c->g.vh, 0); return WL_OK;
     case WL_OPT_LINK_LOAD: *value = c->link_load; return WL_OK;
+    case WL_OPT_LDS_RESERVE: *value = c->g.lds; return WL_OK;
     default: return fail(WL_E_ARG, "unknown option %d", option);
     }
 }

@@ -116,7 +116,7 @@ static hipError_t launch_t(const Geom &g, const DslashLaunch &L, hipStream_t s) 
     A.fin = L.fin;
     const int blk = g.blk;
     const int nb = (g.vh + blk - 1) / blk;
-    hipLaunchKernelGGL((k_dslash<R, SIGN, EPI>), dim3(nb), dim3(blk), 0, s, A);
+    hipLaunchKernelGGL((k_dslash<R, SIGN, EPI>), dim3(nb), dim3(blk), g.lds, s, A);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || (EPI != EPI_AXPY_NORM && EPI != EPI_CGUPD)) return e;
     return launch_finish(L.red, nb, L.fin, L.done, s);

@@ -412,7 +412,7 @@ static hipError_t rb_launch(const Geom &g, const MrLaunch &L, hipStream_t s) {
     const long waves = (long)A.pstride * A.C;
     const int wpb = g.blk / 64;
     hipLaunchKernelGGL((k_dslash_rb<T, SIGN, EPI, NR, NTL>), dim3((unsigned)((waves + wpb - 1) / wpb)),
-                       dim3(g.blk), 0, s, A);
+                       dim3(g.blk), g.lds, s, A);
     return hipGetLastError();
 }
 

@@ -180,6 +180,9 @@ int wl_cg_solve_mixed(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa,
  *       field of the context exists (else WL_E_STATE).
  *   WL_OPT_LINK_LOAD  link loads of the multi-rhs kernel: 0 = automatic (nontemporal when a
  *       launch reads each link once), 1 = cached, 2 = nontemporal.
+ *   WL_OPT_LDS_RESERVE  bytes of LDS reserved by every hopping workgroup (0..65536; 0 = none).
+ *       The kernels use no LDS; the reservation caps how many workgroups share a CU, i.e. the
+ *       sites in flight, whose neighbour spinors must fit the caches to be reused.
  * Results do not depend on these options; only the speed does. */
 enum {
     WL_OPT_TRAVERSAL_CHUNK = 1,
@@ -187,7 +190,8 @@ enum {
     WL_OPT_RHS_BLOCK = 3,
     WL_OPT_RHS_LAUNCH = 4,
     WL_OPT_PLANE_PAD = 5,
-    WL_OPT_LINK_LOAD = 6
+    WL_OPT_LINK_LOAD = 6,
+    WL_OPT_LDS_RESERVE = 7
 };
 int wl_set_option(wl_ctx *ctx, int option, int value);
 int wl_get_option(const wl_ctx *ctx, int option, int *value);
