@@ -263,6 +263,7 @@ int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, c
         // one rhs chunk per lane: nobody reads the link again, stream it; several chunks: the
         // sibling waves of a segment re-read it from the caches
         L.link_nt = c->link_load == 2 || (c->link_load == 0 && L.NB <= nr);
+        L.pipe = c->pipeline == 0;
         L.all_done = all_done;
         L.st = st;
         L.x = x;
@@ -667,6 +668,14 @@ extern "C" int wl_set_option(wl_ctx *c, int option, int value) {
         if (value < 0 || value > 65536) return fail(WL_E_ARG, "LDS reserve must be 0..65536 bytes");
         c->g.lds = value;
         return WL_OK;
+    case WL_OPT_GRAPHS:
+        if (value != 0 && value != 1) return fail(WL_E_ARG, "graphs option must be 0 or 1");
+        c->use_graphs = value == 1;
+        return WL_OK;
+    case WL_OPT_PIPELINE:
+        if (value != 0 && value != 1) return fail(WL_E_ARG, "pipeline option must be 0 or 1");
+        c->pipeline = value;
+        return WL_OK;
     case WL_OPT_PLANE_PAD: {
         if (value != -1 && (value < 0 || value > 65536 || value % 64 != 0))
             return fail(WL_E_ARG, "plane padding must be -1 or a multiple of 64 in 0..65536");
@@ -691,6 +700,8 @@ extern "C" int wl_get_option(const wl_ctx *c, int option, int *value) {
     case WL_OPT_PLANE_PAD: *value = c->g.stride - plane_stride(c->g.vh, 0); return WL_OK;
     case WL_OPT_LINK_LOAD: *value = c->link_load; return WL_OK;
     case WL_OPT_LDS_RESERVE: *value = c->g.lds; return WL_OK;
+    case WL_OPT_GRAPHS: *value = c->use_graphs ? 1 : 0; return WL_OK;
+    case WL_OPT_PIPELINE: *value = c->pipeline; return WL_OK;
     default: return fail(WL_E_ARG, "unknown option %d", option);
     }
 }

@@ -247,7 +247,16 @@ int cg_mixed(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, i
 // ------------------------------------------------------------------ BatchCG ------------
 
 BatchCG::~BatchCG() {
+    drop_graphs();
     for (hipEvent_t e : all_ev_) (void)hipEventDestroy(e);
+}
+
+void BatchCG::drop_graphs() {
+    for (hipGraphExec_t &g : exec_)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
 }
 
 int BatchCG::prepare(wl_ctx *c, int N, int prec, bool mixed) {
@@ -261,7 +270,8 @@ int BatchCG::prepare(wl_ctx *c, int N, int prec, bool mixed) {
         (rc = work_field(c, W_DDP64, WL_F64, WL_EVEN, N, &f)) ||
         (rc = work_field(c, W_T64, WL_F64, WL_ODD, N, &f)))
         return rc;
-    (void)mixed;
+    if (mixed && prec != WL_F64)
+        if ((rc = work_field(c, W_Y, WL_F32, WL_EVEN, N, &f))) return rc;
     const size_t rows = (size_t)std::max(nblocks(c->g), mr_segments(c->g));
     if ((rc = ensure_partials(c, (size_t)N * rows))) return rc;
     return ensure_staging(c, (size_t)c->g.vh * 24 * sizeof(double));
@@ -289,6 +299,8 @@ int BatchCG::begin(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec)
     chunks_ = 0;
     stop_ = final_enqueued_ = done_ = false;
     inflight_.clear();
+    drop_graphs();  // operands or shapes may differ from the previous solve
+    use_graphs_ = c->use_graphs;
     t0_ = std::chrono::steady_clock::now();
     int rc;
     if ((rc = prepare(c, N, s_.prec, s_.mixed))) return rc;
@@ -299,6 +311,12 @@ int BatchCG::begin(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec)
         (rc = work_field(c, W_DDP64, WL_F64, WL_EVEN, N, &DDP64_)) ||
         (rc = work_field(c, W_T64, WL_F64, WL_ODD, N, &T64_)))
         return rc;
+    Y_ = nullptr;
+    acc_ = x;
+    if (s_.mixed && prec != WL_F64) {
+        if ((rc = work_field(c, W_Y, WL_F32, WL_EVEN, N, &Y_))) return rc;
+        acc_ = Y_->data;
+    }
     pplain_ = nblocks(c->g);
     cap_chunks_ = s_.maxiters > 0 ? (s_.maxiters + kChunk - 1) / kChunk + 4 : 0;
 
@@ -314,6 +332,7 @@ int BatchCG::begin(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec)
     hipStream_t S = c->stream;
     HIPCHK(hipMemcpyAsync(st, c->st_mr_host, N * sizeof(CgState), hipMemcpyHostToDevice, S));
     HIPCHK(hipMemsetAsync(c->flags_d, 0, 64, S));
+    if (Y_) HIPCHK(hipMemsetAsync(Y_->data, 0, (size_t)N * Y_->rhs_elems * 8, S));
     double *part = c->partials;
     HIPCHK(launch_norm2_mr(WL_F64, c->g, b, N, part, S));
     HIPCHK(launch_finish_mr(part, pplain_, pplain_, N, st, nullptr, FIN_SOURCE, nullptr, S));
@@ -337,9 +356,9 @@ int BatchCG::begin(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec)
     return pump(&fin);
 }
 
-// kChunk iterations (conjugate_gradient.h:75-95 per rhs), the gated reliable update, and a host
-// check of the all-finished flag.
-int BatchCG::enqueue_chunk() {
+// kChunk iterations (conjugate_gradient.h:75-95 per rhs), the gated reliable update, and the
+// copy of the all-finished flag to host slot `slot`.
+int BatchCG::enqueue_chunk_body(int slot) {
     wl_ctx *c = c_;
     hipStream_t S = c->stream;
     CgState *st = c->st_mr;
@@ -357,7 +376,7 @@ int BatchCG::enqueue_chunk() {
                                 st, nullptr, nullptr, nullptr, nullptr)))
             return rc;
         if ((rc = dslash_mr_all(c, prec, nullptr, 0, T_->data, DP_->data, s_.kappa, -1, EPI_CGUPD, N, all_done,
-                                st, x_, P_->data, R_->data, &ps)))
+                                st, acc_, P_->data, R_->data, &ps)))
             return rc;
         HIPCHK(launch_finish_mr(c->partials, ps, ps, N, st, nullptr, s_.mixed ? FIN_RRNEW_MIXED2 : FIN_RRNEW,
                                 all_done, S));
@@ -367,9 +386,38 @@ int BatchCG::enqueue_chunk() {
     }
     if (s_.mixed)
         if ((rc = enqueue_update())) return rc;
-    const int slot = chunks_ & 3;
-    ++chunks_;
     HIPCHK(hipMemcpyAsync(c->flags_h + 4 * slot, c->flags_d, 2 * sizeof(int), hipMemcpyDeviceToHost, S));
+    return WL_OK;
+}
+
+int BatchCG::enqueue_chunk() {
+    wl_ctx *c = c_;
+    hipStream_t S = c->stream;
+    const int slot = chunks_ & 1;  // at most two chunks in flight
+    int rc;
+    if (use_graphs_) {
+        if (!exec_[slot]) {
+            HIPCHK(hipStreamBeginCapture(S, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_chunk_body(slot);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(S, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(WL_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+            const hipError_t ei = hipGraphInstantiate(&exec_[slot], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                exec_[slot] = nullptr;
+                return fail(WL_E_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+            }
+        }
+        HIPCHK(hipGraphLaunch(exec_[slot], S));
+    } else if ((rc = enqueue_chunk_body(slot))) {
+        return rc;
+    }
+    ++chunks_;
     hipEvent_t ev;
     if ((rc = get_event(&ev))) return rc;
     HIPCHK(hipEventRecord(ev, S));
@@ -387,6 +435,7 @@ int BatchCG::enqueue_update() {
     const int *gate = c->flags_d + 2;
     const int N = N_, prec = s_.prec;
     int rc;
+    if (Y_) HIPCHK(launch_fold_mr(c->g, x_, Y_->data, st, 0, N, gate, S));
     if ((rc = meo_mr(c, WL_F64, DP64_->data, x_, T64_->data, s_.kappa, +1, EPI_AXPY, N, gate, nullptr, nullptr)))
         return rc;
     if ((rc = meo_mr(c, WL_F64, DDP64_->data, DP64_->data, T64_->data, s_.kappa, -1, EPI_AXPY, N, gate, nullptr,
@@ -406,6 +455,7 @@ int BatchCG::enqueue_final() {
     CgState *st = c->st_mr;
     const int N = N_;
     int rc;
+    if (Y_) HIPCHK(launch_fold_mr(c->g, x_, Y_->data, st, 1, N, nullptr, S));
     if ((rc = meo_mr(c, WL_F64, DP64_->data, x_, T64_->data, s_.kappa, +1, EPI_AXPY, N, nullptr, nullptr, nullptr)))
         return rc;
     if ((rc = meo_mr(c, WL_F64, DDP64_->data, DP64_->data, T64_->data, s_.kappa, -1, EPI_AXPY, N, nullptr, nullptr,

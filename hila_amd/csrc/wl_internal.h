@@ -94,9 +94,10 @@ struct MrLaunch {
     int out_par, sign, epi;
     int j0, NB, NR;
     bool link_nt;             // nontemporal link loads
+    bool pipe;                // software-pipelined hops where the registers allow
     const int *all_done;      // may be null
     CgState *st;              // per-rhs states (skip finished rhs, alpha); may be null
-    void *x;
+    void *x;                  // EPI_CGUPD accumulator: FP64 x (FP64 arithmetic) or FP32 y
     const void *p;
     void *r;
     double *partials;
@@ -119,6 +120,9 @@ hipError_t launch_xpay_mr(int prec, const Geom &g, void *p, const void *r, const
 hipError_t launch_batch_sync(CgState *st, int N, int mixed, int *flags, const int *gate,
                              hipStream_t s);
 hipError_t launch_zero_rhs(const Geom &g, void *x, int j, hipStream_t s);
+// x_j += y_j, y_j = 0 (x FP64, y FP32) for rhs taking a reliable update, or all with all != 0
+hipError_t launch_fold_mr(const Geom &g, void *x, void *y, const CgState *st, int all, int N, const int *gate,
+                          hipStream_t s);
 
 // one-workgroup kernel: sum nparts partials in order, apply `fin` (skipped when *done != 0)
 hipError_t launch_finish(const RedArgs &ra, int nparts, int fin, const int *done, hipStream_t s);
@@ -179,6 +183,8 @@ struct wl_ctx {
     int rhs_block = 0;                 // rhs per lane of the multi-rhs kernel (0: automatic)
     int rhs_launch = 0;                // rhs per launch (pass over the lattice; 0: automatic)
     int link_load = 0;                 // multi-rhs link loads: 0 automatic, 1 cached, 2 nontemporal
+    bool use_graphs = false;           // batched CG: replay each chunk from a captured HIP graph
+    int pipeline = 0;                  // multi-rhs hops: 0 pipelined where possible, 1 never
     // fields created through wl_field_create and not yet destroyed; wl_destroy orphans them
     std::unordered_set<wl_field *> user_fields;
     size_t dev_bytes = 0;              // device memory held by the context and its fields

@@ -83,6 +83,7 @@ size_t slot_estimate(const int dims[4], int prec, int cap) {
     size_t bytes = (size_t)cap * (2 * f64          // x, b
                                   + 4 * flo        // r, p, Dp (even), t (odd)
                                   + 3 * f64);      // FP64 update work
+    if (prec != WL_F64) bytes += (size_t)cap * 12 * S * 8;  // FP32 inner solution
     bytes += (size_t)72 * S * 16;                  // FP64 gauge
     if (prec != WL_F64) bytes += (size_t)72 * S * elem_bytes(prec);
     bytes += (size_t)V * 18 * 8;                   // staging (gauge upload)
@@ -117,6 +118,8 @@ struct wl_server {
     uint64_t generation = 0;  // completed requests so far (wakes a blocked loader)
     bool stop = false;        // no more submissions; the loader exits once the queue is empty
     bool stop_sched = false, stop_unload = false;
+    bool submitted = false;                     // options are frozen after the first submission
+    std::vector<std::pair<int, int>> slot_opts; // wl_set_option calls applied to every new slot
     std::thread loader, scheduler, unloader;
 };
 
@@ -170,7 +173,8 @@ void destroy_slot(Slot *s) {
     s->ctx = nullptr;
 }
 
-int create_slot(wl_server *S, const wl_solve_request &r, int cap, std::unique_ptr<Slot> *out) {
+int create_slot(wl_server *S, const wl_solve_request &r, int cap,
+                const std::vector<std::pair<int, int>> &opts, std::unique_ptr<Slot> *out) {
     auto s = std::make_unique<Slot>();
     std::memcpy(s->dims, r.dims, sizeof(s->dims));
     s->antiperiodic = r.antiperiodic_t ? 1 : 0;
@@ -179,6 +183,11 @@ int create_slot(wl_server *S, const wl_solve_request &r, int cap, std::unique_pt
     int rc = wl_create(r.dims, s->antiperiodic, S->device, &s->ctx);
     if (rc) return rc;
     wl_ctx *c = s->ctx;
+    for (const auto &o : opts)
+        if ((rc = wl_set_option(c, o.first, o.second))) {
+            destroy_slot(s.get());
+            return rc;
+        }
     if ((rc = field_new(c, WL_EVEN, WL_F64, cap, &s->x)) || (rc = field_new(c, WL_EVEN, WL_F64, cap, &s->b))) {
         destroy_slot(s.get());
         return rc;
@@ -260,8 +269,9 @@ void loader_main(wl_server *S) {
                 continue;
             }
             std::unique_ptr<Slot> sp;
+            const std::vector<std::pair<int, int>> opts = S->slot_opts;
             lk.unlock();
-            int rc = create_slot(S, r, cap, &sp);
+            int rc = create_slot(S, r, cap, opts, &sp);
             lk.lock();
             if (rc) {
                 S->pending.pop_front();
@@ -472,6 +482,7 @@ extern "C" int wl_server_submit(wl_server *S, const wl_solve_request *req, uint6
         std::lock_guard<std::mutex> lk(S->mu);
         if (S->stop) return fail(WL_E_STATE, "server is shutting down");
         q->ticket = S->next_ticket++;
+        S->submitted = true;
         S->requests[q->ticket] = q;
         S->pending.push_back(q);
         *ticket = q->ticket;
@@ -511,6 +522,15 @@ extern "C" int wl_server_info_get(wl_server *S, wl_server_info *info) {
     info->completed = S->completed;
     info->slots_created = S->slots_created;
     info->slots_released = S->slots_released;
+    return WL_OK;
+}
+
+extern "C" int wl_server_set_option(wl_server *S, int option, int value) {
+    if (!S) return fail(WL_E_ARG, "null server");
+    if (option < WL_OPT_TRAVERSAL_CHUNK || option > WL_OPT_PIPELINE) return fail(WL_E_ARG, "unknown option %d", option);
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->submitted) return fail(WL_E_STATE, "server options must be set before the first submission");
+    S->slot_opts.emplace_back(option, value);
     return WL_OK;
 }
 

@@ -4,13 +4,17 @@
 // host checks and enqueues more work, so that one thread can drive many solves on many
 // streams (the request server) or one solve to completion (wl_cg_solve for nrhs > 1).
 //
-// Work is enqueued in chunks of kChunk iterations.  Every kernel of an iteration reads a
+// Work is enqueued in chunks of kChunk iterations.  With WL_OPT_GRAPHS a chunk (its kernels
+// and the flag copy; no memsets) is captured once per solve into a HIP graph and replayed,
+// one launch per chunk instead of ~75.  Every kernel of an iteration reads a
 // device flag and returns at once when all right-hand sides have finished, and rhs that have
 // finished (or wait for a reliable update) are skipped inside the kernels, so chunks can be
 // enqueued ahead of the host's knowledge of convergence: up to `depth` chunks are in flight.
-// In mixed precision a chunk ends with the reliable-update sequence (FP64 M^dag M x, new
-// residual, restart), gated on the device by the update-pending flag, so that the host takes
-// no decision inside the iteration either.
+// In mixed precision a chunk ends with the reliable-update sequence (fold the inner solution
+// into x, FP64 M^dag M x, new residual, restart), gated on the device by the update-pending
+// flag, so that the host takes no decision inside the iteration either.  With an FP32 or FP16
+// inner precision the iterations accumulate alpha p into an FP32 inner solution y, which each
+// reliable update adds to the FP64 x (and clears): the iterations then never touch x.
 #pragma once
 
 #include <chrono>
@@ -62,9 +66,11 @@ class BatchCG {
         bool final;  // final states copy
     };
     int enqueue_chunk();
+    int enqueue_chunk_body(int slot);
     int enqueue_update();
     int enqueue_final();
     int get_event(hipEvent_t *ev);
+    void drop_graphs();
 
     wl_ctx *c_ = nullptr;
     void *x_ = nullptr;
@@ -73,6 +79,8 @@ class BatchCG {
     CgSpec s_{};
     wl_field *R_ = nullptr, *P_ = nullptr, *DP_ = nullptr, *T_ = nullptr;
     wl_field *DP64_ = nullptr, *DDP64_ = nullptr, *T64_ = nullptr;
+    wl_field *Y_ = nullptr;  // FP32 inner solution (FP32 / FP16 inner precision), folded into x
+    void *acc_ = nullptr;    // what x += alpha p accumulates into: Y_ or x itself (FP64)
     int pplain_ = 0;
     int chunks_ = 0;     // chunks enqueued
     int cap_chunks_ = 0;
@@ -85,6 +93,9 @@ class BatchCG {
     std::vector<hipEvent_t> all_ev_;
     std::chrono::steady_clock::time_point t0_;
     double ms_ = 0;
+    // chunk graphs (WL_OPT_GRAPHS): one per flags slot parity, captured on first use in a solve
+    bool use_graphs_ = false;
+    hipGraphExec_t exec_[2] = {nullptr, nullptr};
 };
 
 }  // namespace wl

@@ -66,6 +66,29 @@ __device__ __forceinline__ cx<float> lde(const h2 *p) {
     return {(float)v.x, (float)v.y};
 }
 
+// One stored element as it is in memory (kept packed in registers until used), and its
+// widening to the arithmetic type.
+template <bool NT>
+__device__ __forceinline__ cx<double> ldraw(const cx<double> *p) {
+    return ldc<NT>(p);
+}
+template <bool NT>
+__device__ __forceinline__ cx<float> ldraw(const cx<float> *p) {
+    return ldc<NT>(p);
+}
+template <bool NT>
+__device__ __forceinline__ h2 ldraw(const h2 *p) {
+    unsigned bits;
+    if constexpr (NT)
+        bits = __builtin_nontemporal_load((const unsigned *)p);
+    else
+        bits = *(const unsigned *)p;
+    return __builtin_bit_cast(h2, bits);
+}
+__device__ __forceinline__ cx<double> widen(cx<double> v) { return v; }
+__device__ __forceinline__ cx<float> widen(cx<float> v) { return v; }
+__device__ __forceinline__ cx<float> widen(h2 v) { return {(float)v.x, (float)v.y}; }
+
 // FP32 scale plane of the rhs block starting at `base` (f16s only).
 __device__ __forceinline__ const float *scale_plane(const h2 *base, int S) {
     return reinterpret_cast<const float *>(base + (size_t)12 * S);

@@ -29,7 +29,7 @@ WL_OK, WL_E_ARG, WL_E_OOM, WL_E_DEVICE, WL_E_NOCONV, WL_E_STATE, WL_E_NOTIMPL, W
 EVEN, ODD = 1, 2
 F64, F32, F16 = 0, 1, 2
 OPT_TRAVERSAL_CHUNK, OPT_BLOCK, OPT_RHS_BLOCK, OPT_RHS_LAUNCH, OPT_PLANE_PAD, OPT_LINK_LOAD = 1, 2, 3, 4, 5, 6
-OPT_LDS_RESERVE = 7
+OPT_LDS_RESERVE, OPT_GRAPHS, OPT_PIPELINE = 7, 8, 9
 
 EXPORTED_SYMBOLS = [
     "wl_version", "wl_last_error", "wl_create", "wl_destroy", "wl_sync", "wl_volume",
@@ -37,7 +37,7 @@ EXPORTED_SYMBOLS = [
     "wl_field_download", "wl_field_zero", "wl_field_copy", "wl_field_norm2", "wl_dslash_eo", "wl_meo_apply",
     "wl_cg_solve", "wl_cg_solve_mixed", "wl_set_option", "wl_get_option", "wl_time_op", "wl_gen_gauge", "wl_gen_spinor", "wl_gen_point",
     "wl_server_create", "wl_server_submit", "wl_server_wait", "wl_server_info_get", "wl_server_destroy",
-    "wl_host_alloc", "wl_host_free", "wl_autotune",
+    "wl_host_alloc", "wl_host_free", "wl_autotune", "wl_server_set_option",
 ]
 
 
@@ -136,6 +136,7 @@ def lib():
         L.wl_host_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
         L.wl_host_free.argtypes = [vp]
         L.wl_autotune.argtypes = [vp, i, i, C.POINTER(TuneResult)]
+        L.wl_server_set_option.argtypes = [vp, i, i]
         for name in EXPORTED_SYMBOLS:
             if name not in ("wl_version", "wl_last_error", "wl_volume"):
                 getattr(L, name).restype = C.c_int
@@ -470,6 +471,10 @@ class Server:
         check(lib().wl_server_wait(self._h, int(ticket), int(timeout_ms), C.byref(res), stats))
         self._keep.pop(ticket, None)
         return res.as_dict(), [s.as_dict() for s in stats[: res.nrhs]]
+
+    def set_option(self, option, value):
+        """A kernel option (OPT_*) for every slot; only before the first submission."""
+        check(lib().wl_server_set_option(self._h, int(option), int(value)))
 
     def info(self):
         i = ServerInfo()

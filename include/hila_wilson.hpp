@@ -185,9 +185,22 @@ class CG {
     // out: initial guess on entry, solution on exit.  Reaching maxiters is not an error (as in
     // HILA); the statistics of the last call are kept in `stats`.
     void apply(vector_type &in, vector_type &out) {
-        const int rc = wl_cg_solve(M.gauge().lattice().ctx(), out.part(EVEN), in.part(EVEN),
-                                   M.kappa, accuracy, (int)maxiters, &stats);
+        wl_ctx *c = M.gauge().lattice().ctx();
+        const int rc = inner_prec_ == WL_F64
+                           ? wl_cg_solve(c, out.part(EVEN), in.part(EVEN), M.kappa, accuracy,
+                                         (int)maxiters, &stats)
+                           : wl_cg_solve_mixed(c, out.part(EVEN), in.part(EVEN), M.kappa, accuracy,
+                                               (int)maxiters, inner_prec_, delta_, update_every_,
+                                               &stats);
         if (rc != WL_E_NOCONV) check(rc, "wl_cg_solve");
+    }
+    // Mixed precision (not in HILA's CG): inner operator and Krylov vectors in WL_F32 or WL_F16
+    // (with per-site scale), FP64 reliable updates when ||r|| drops by delta or every
+    // update_every iterations, convergence on the FP64 true residual.  WL_F64 restores HILA's CG.
+    void set_precision(int inner_prec, double delta = 1e-3, int update_every = 100) {
+        inner_prec_ = inner_prec;
+        delta_ = delta;
+        update_every_ = update_every;
     }
     wl_cg_stats stats{};
 
@@ -195,6 +208,35 @@ class CG {
     Op &M;
     double accuracy = 1e-12;   // CG_DEFAULT_ACCURACY (conjugate_gradient.h:16)
     double maxiters = 10000;   // CG_DEFAULT_MAXITERS (conjugate_gradient.h:15)
+    int inner_prec_ = WL_F64;
+    double delta_ = 1e-3;
+    int update_every_ = 100;
+};
+
+// Request server (wl_server_*): independent solves admitted by HBM budget and run
+// concurrently on one GPU.  Host buffers of a request must outlive its wait().
+class Server {
+  public:
+    explicit Server(int device = 0, size_t hbm_budget_bytes = 0, int max_active = 0) {
+        check(wl_server_create(device, hbm_budget_bytes, max_active, &srv_), "wl_server_create");
+    }
+    ~Server() { wl_server_destroy(srv_); }
+    Server(const Server &) = delete;
+    Server &operator=(const Server &) = delete;
+    uint64_t submit(const wl_solve_request &r) {
+        uint64_t t = 0;
+        check(wl_server_submit(srv_, &r, &t), "wl_server_submit");
+        return t;
+    }
+    // Blocks until the request finishes; per_rhs (if given) receives r.nrhs entries.
+    wl_solve_result wait(uint64_t ticket, wl_cg_stats *per_rhs = nullptr) {
+        wl_solve_result res{};
+        check(wl_server_wait(srv_, ticket, -1, &res, per_rhs), "wl_server_wait");
+        return res;
+    }
+
+  private:
+    wl_server *srv_ = nullptr;
 };
 
 }  // namespace hila_wilson

@@ -183,6 +183,11 @@ int wl_cg_solve_mixed(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa,
  *   WL_OPT_LDS_RESERVE  bytes of LDS reserved by every hopping workgroup (0..65536; 0 = none).
  *       The kernels use no LDS; the reservation caps how many workgroups share a CU, i.e. the
  *       sites in flight, whose neighbour spinors must fit the caches to be reused.
+ *   WL_OPT_GRAPHS  1: the batched CG (nrhs > 1, FP16 inner, the server) captures each chunk
+ *       of iterations into a HIP graph once per solve and replays it; 0 (default): launches
+ *       kernel by kernel.  Results are identical either way.
+ *   WL_OPT_PIPELINE  multi-rhs kernel: 0 (default) = software-pipelined hops where the
+ *       registers allow (FP32 and FP16 storage, up to 2 rhs per lane); 1 = never.
  * Results do not depend on these options; only the speed does. */
 enum {
     WL_OPT_TRAVERSAL_CHUNK = 1,
@@ -191,7 +196,9 @@ enum {
     WL_OPT_RHS_LAUNCH = 4,
     WL_OPT_PLANE_PAD = 5,
     WL_OPT_LINK_LOAD = 6,
-    WL_OPT_LDS_RESERVE = 7
+    WL_OPT_LDS_RESERVE = 7,
+    WL_OPT_GRAPHS = 8,
+    WL_OPT_PIPELINE = 9
 };
 int wl_set_option(wl_ctx *ctx, int option, int value);
 int wl_get_option(const wl_ctx *ctx, int option, int *value);
@@ -291,6 +298,11 @@ int wl_server_submit(wl_server *srv, const wl_solve_request *req, uint64_t *tick
 int wl_server_wait(wl_server *srv, uint64_t ticket, int timeout_ms, wl_solve_result *res,
                    wl_cg_stats *per_rhs);
 int wl_server_info_get(wl_server *srv, wl_server_info *info);
+/* A kernel option (WL_OPT_*, as wl_set_option) for every slot of the server; only before the
+ * first submission (else WL_E_STATE).  The multi-rhs kernel shape (WL_OPT_RHS_BLOCK,
+ * WL_OPT_RHS_LAUNCH, WL_OPT_BLOCK) is taken per request from wl_autotune's remembered choice
+ * for its lattice size, precision and rhs count, or the defaults. */
+int wl_server_set_option(wl_server *srv, int option, int value);
 /* Finish every submitted request, then release all device memory and threads. */
 int wl_server_destroy(wl_server *srv);
 

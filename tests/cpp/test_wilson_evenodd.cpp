@@ -92,6 +92,19 @@ static void run(Lattice &lattice, int link_kind, const char *name) {
     CHECK(inverse.stats.converged == 1 && inverse.stats.true_resid < 1e-11,
           "CG true residual below accuracy");
 
+    // the same round trip with the mixed-precision inverter (FP32 and FP16 inner, FP64
+    // reliable updates): the true residual meets the accuracy
+    for (int prec : {WL_F32, WL_F16}) {
+        CG<dirac> mixed(D, 1e-10, 10000);
+        mixed.set_precision(prec, 0.1, 100);
+        a.zero(ALL);
+        mixed.apply(DdaggerDb, a);
+        diffre = diff2_even(a, b);
+        CHECK(mixed.stats.converged == 1 && mixed.stats.true_resid <= 1e-10,
+              prec == WL_F32 ? "mixed CG (FP32 inner) true residual" : "mixed CG (FP16 inner) true residual");
+        CHECK(diffre < 1e-14 * b.squarenorm(EVEN), "mixed (DdgD)^-1 DdgD");
+    }
+
     // apply leaves the odd half of the output at zero (wilson.h:278)
     Db.gaussian_random(16, ALL);
     D.apply(b, Db);

@@ -181,3 +181,18 @@ def test_pinned_request_buffers():
     assert np.array_equal(xb.array, sol_a)
     for b in (gb, sb, xb):
         b.close()
+
+
+def test_server_with_graph_replay_matches():
+    reqs = [MIX[1], MIX[3], MIX[6]]
+    base = [solve_alone(r) for r in reqs]
+    with W.Server(max_active=4) as S:
+        S.set_option(W.OPT_GRAPHS, 1)
+        subs = [r.submit(S) for r in reqs]
+        outs = [(S.wait(t), sol) for t, sol in subs]
+        with pytest.raises(W.WilsonError) as e:
+            S.set_option(W.OPT_GRAPHS, 0)  # frozen after the first submission
+        assert e.value.code == W.WL_E_STATE
+    for ((res, st), sol), (bres, bst, bsol) in zip(outs, base):
+        assert res["status"] == W.WL_OK
+        assert np.array_equal(sol, bsol)
