@@ -388,6 +388,45 @@ def serving_config(W, n_poisson=60, n_closed=24, rate=20.0, per_volume=3, seed=1
     }
 
 
+def profile_multi(W, steps):
+    """Autotuned multi-rhs hopping launches (12 rhs at 32^4, links exp(i 0.3 H) seed 5,
+    Gaussian sources), FP16 then FP32 storage, then copies of a 12-rhs FP16 field (the byte
+    counters' calibration: 12 x 13 planes x stride x 4 B each way)."""
+    dims, N = (32, 32, 32, 32), 12
+    lat = W.Lattice(dims)
+    lat.upload_gauge(W.gen_gauge(dims, "smooth", 0.3, 5))
+    D = W.Dirac_Wilson_evenodd(0.125, lat)
+    src = W.gen_spinor(dims, 10, W.EVEN)
+    for prec in (W.F16, W.F32):
+        fin = W.Field(lat, W.EVEN, prec, nrhs=N)
+        for j in range(N):
+            fin.upload(src, rhs=j)
+        fo = W.Field(lat, W.ODD, prec, nrhs=N)
+        tune = lat.autotune(prec, N)
+        print(f"# profile-multi: prec {prec} tuned {tune}", file=sys.stderr, flush=True)
+        for _ in range(steps):
+            D.hop_set(fin, fo, 1)
+        lat.sync()
+        if prec == W.F16:
+            fc = W.Field(lat, W.EVEN, prec, nrhs=N)
+            for _ in range(steps):
+                fc.copy_from(fin)
+            lat.sync()
+    lat.close()
+
+
+def profile_serve(W):
+    """The closed batch of the serving configuration (24 requests), FP16 inner."""
+    rng = np.random.default_rng(11)
+    gauges, sources = serving_pool(W, rng, 3)
+    pool = _SolutionPool(W)
+    closed = draw_requests(rng, 24, 3)
+    with W.Server(max_active=16) as S:
+        res, wall = run_served(W, S, closed, gauges, sources, pool, precision="f16")
+    print(f"# profile-serve: {len(closed)} requests, {sum(r['nrhs'] for r in closed)} rhs in "
+          f"{wall:.2f} s", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(U, psi, seconds=15.0):
     """The oracle's HILA-structured hopping term (projection pass + per-direction
     accumulation, wilson.h:53-95) on the host cores, on a bounded number of applications."""
@@ -427,6 +466,13 @@ def main(argv=None):
                     help="skip the batched multi-rhs configuration")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving configuration")
     ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--profile-multi", action="store_true",
+                    help="short run for rocprofv3: --steps autotuned multi-rhs hopping launches "
+                         "(12 rhs, FP16 then FP32 storage) at 32^4, then --steps copies of one "
+                         "12-rhs FP16 field; prints no JSON line")
+    ap.add_argument("--profile-serve", action="store_true",
+                    help="short run for rocprofv3: the closed batch of 24 served requests, FP16 "
+                         "inner; prints no JSON line")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: --steps hopping launches, then --steps "
                          "plain field copies (12 planes of V/2 complex doubles each way) as "
@@ -438,6 +484,13 @@ def main(argv=None):
         sys.exit(2)
 
     from hila_amd import wilson as W
+
+    if a.profile_multi:
+        profile_multi(W, a.steps)
+        return
+    if a.profile_serve:
+        profile_serve(W)
+        return
 
     t_setup = time.perf_counter()
     U = W.gen_gauge(DIMS, "smooth", 0.3, 3)

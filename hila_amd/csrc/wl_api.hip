@@ -218,10 +218,12 @@ int work_field(wl_ctx *c, int slot, int prec, int parity, int nrhs, wl_field **o
     return rc;
 }
 
-// Right-hand sides per lane of the multi-rhs kernel (register blocking).
+// Right-hand sides per lane of the multi-rhs kernel (register blocking): at most 2 in FP64
+// (3-4 spill) and 4 with FP32 arithmetic.  Defaults from the served request mix on MI355X:
+// 4 for FP32 storage, 2 for FP64 and FP16 (FP16 at 4 per lane spills in the pipelined form).
 int rhs_per_lane(const wl_ctx *c, int prec, int N) {
-    const int cap = prec == WL_F64 ? 2 : 4;  // FP32 arithmetic (FP32 and FP16 storage): 4
-    const int nr = c->rhs_block > 0 ? c->rhs_block : cap;
+    const int cap = prec == WL_F64 ? 2 : 4;
+    const int nr = c->rhs_block > 0 ? c->rhs_block : (prec == WL_F32 ? 4 : 2);
     return std::max(1, std::min(nr, std::min(N, cap)));
 }
 
@@ -263,6 +265,7 @@ int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, c
         // one rhs chunk per lane: nobody reads the link again, stream it; several chunks: the
         // sibling waves of a segment re-read it from the caches
         L.link_nt = c->link_load == 2 || (c->link_load == 0 && L.NB <= nr);
+        L.lds_links = c->link_load == 3;
         L.pipe = c->pipeline == 0;
         L.all_done = all_done;
         L.st = st;
@@ -661,7 +664,7 @@ extern "C" int wl_set_option(wl_ctx *c, int option, int value) {
         c->rhs_launch = value;
         return WL_OK;
     case WL_OPT_LINK_LOAD:
-        if (value < 0 || value > 2) return fail(WL_E_ARG, "link load mode must be 0, 1 or 2");
+        if (value < 0 || value > 3) return fail(WL_E_ARG, "link load mode must be 0..3");
         c->link_load = value;
         return WL_OK;
     case WL_OPT_LDS_RESERVE:

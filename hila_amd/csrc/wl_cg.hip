@@ -456,14 +456,21 @@ int BatchCG::enqueue_final() {
     const int N = N_;
     int rc;
     if (Y_) HIPCHK(launch_fold_mr(c->g, x_, Y_->data, st, 1, N, nullptr, S));
-    if ((rc = meo_mr(c, WL_F64, DP64_->data, x_, T64_->data, s_.kappa, +1, EPI_AXPY, N, nullptr, nullptr, nullptr)))
+    // In mixed precision an rhs converges only at a reliable update, which has just computed
+    // its FP64 true residual for the final x: the recomputation runs only if some rhs ended
+    // otherwise (maxiters).  Plain CG always recomputes it.
+    const int *skip = nullptr;
+    if (s_.mixed) {
+        HIPCHK(launch_all_converged(st, N, c->flags_d + 3, S));
+        skip = c->flags_d + 3;
+    }
+    if ((rc = meo_mr(c, WL_F64, DP64_->data, x_, T64_->data, s_.kappa, +1, EPI_AXPY, N, skip, nullptr, nullptr)))
         return rc;
-    if ((rc = meo_mr(c, WL_F64, DDP64_->data, DP64_->data, T64_->data, s_.kappa, -1, EPI_AXPY, N, nullptr, nullptr,
+    if ((rc = meo_mr(c, WL_F64, DDP64_->data, DP64_->data, T64_->data, s_.kappa, -1, EPI_AXPY, N, skip, nullptr,
                      nullptr)))
         return rc;
-    HIPCHK(launch_residual_update_mr(WL_F64, c->g, b_, DDP64_->data, DP64_->data, st, 1, N, c->partials, nullptr,
-                                     S));
-    HIPCHK(launch_finish_mr(c->partials, pplain_, pplain_, N, st, nullptr, FIN_TRUE, nullptr, S));
+    HIPCHK(launch_residual_update_mr(WL_F64, c->g, b_, DDP64_->data, DP64_->data, st, 1, N, c->partials, skip, S));
+    HIPCHK(launch_finish_mr(c->partials, pplain_, pplain_, N, st, nullptr, FIN_TRUE, skip, S));
     HIPCHK(hipMemcpyAsync(c->st_mr_host, st, N * sizeof(CgState), hipMemcpyDeviceToHost, S));
     hipEvent_t ev;
     if ((rc = get_event(&ev))) return rc;

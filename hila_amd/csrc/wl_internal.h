@@ -94,6 +94,7 @@ struct MrLaunch {
     int out_par, sign, epi;
     int j0, NB, NR;
     bool link_nt;             // nontemporal link loads
+    bool lds_links;           // links staged in LDS, one workgroup per segment (<= 8 rhs chunks)
     bool pipe;                // software-pipelined hops where the registers allow
     const int *all_done;      // may be null
     CgState *st;              // per-rhs states (skip finished rhs, alpha); may be null
@@ -120,6 +121,8 @@ hipError_t launch_xpay_mr(int prec, const Geom &g, void *p, const void *r, const
 hipError_t launch_batch_sync(CgState *st, int N, int mixed, int *flags, const int *gate,
                              hipStream_t s);
 hipError_t launch_zero_rhs(const Geom &g, void *x, int j, hipStream_t s);
+// *flag = 1 if every one of the N rhs has converged (done == 1), else 0
+hipError_t launch_all_converged(const CgState *st, int N, int *flag, hipStream_t s);
 // x_j += y_j, y_j = 0 (x FP64, y FP32) for rhs taking a reliable update, or all with all != 0
 hipError_t launch_fold_mr(const Geom &g, void *x, void *y, const CgState *st, int all, int N, const int *gate,
                           hipStream_t s);
@@ -178,11 +181,12 @@ struct wl_ctx {
     wl::CgState *st_mr = nullptr;      // per-rhs CG states (device) and pinned host mirror
     wl::CgState *st_mr_host = nullptr;
     int st_mr_cap = 0;
-    int *flags_d = nullptr;            // batched CG: [0] all finished, [1] update pending
+    int *flags_d = nullptr;            // batched CG: [0] all finished, [1] update pending,
+                                       // [2] no update pending, [3] all converged
     int *flags_h = nullptr;
     int rhs_block = 0;                 // rhs per lane of the multi-rhs kernel (0: automatic)
     int rhs_launch = 0;                // rhs per launch (pass over the lattice; 0: automatic)
-    int link_load = 0;                 // multi-rhs link loads: 0 automatic, 1 cached, 2 nontemporal
+    int link_load = 0;                 // multi-rhs links: 0 automatic, 1 cached, 2 nontemporal, 3 LDS
     bool use_graphs = false;           // batched CG: replay each chunk from a captured HIP graph
     int pipeline = 0;                  // multi-rhs hops: 0 pipelined where possible, 1 never
     // fields created through wl_field_create and not yet destroyed; wl_destroy orphans them

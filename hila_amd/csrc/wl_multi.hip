@@ -124,6 +124,16 @@ __global__ __launch_bounds__(kBlock) void k_fold_mr(cx<double> *__restrict__ x, 
     }
 }
 
+// flag = 1 when every rhs has converged (done == 1), else 0 (one workgroup of 64).
+__global__ __launch_bounds__(64) void k_all_converged(const CgState *st, int N, int *flag) {
+    __shared__ int all;
+    if (threadIdx.x == 0) all = 1;
+    __syncthreads();
+    if ((int)threadIdx.x < N && st[threadIdx.x].done != 1) atomicAnd(&all, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) *flag = all;
+}
+
 // x_j = 0 for one rhs j
 __global__ __launch_bounds__(kBlock) void k_zero_rhs(cx<double> *__restrict__ x, int j, int vh,
                                                      int S) {
@@ -292,6 +302,11 @@ hipError_t launch_fold_mr(const Geom &g, void *x, void *y, const CgState *st, in
                           hipStream_t s) {
     hipLaunchKernelGGL(k_fold_mr, dim3(nblocks(g), N), dim3(kBlock), 0, s, (cx<double> *)x, (cx<float> *)y, st, all,
                        g.vh, g.stride, gate);
+    return hipGetLastError();
+}
+
+hipError_t launch_all_converged(const CgState *st, int N, int *flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_all_converged, dim3(1), dim3(64), 0, s, st, N, flag);
     return hipGetLastError();
 }
 

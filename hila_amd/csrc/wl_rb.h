@@ -137,16 +137,12 @@ __device__ __forceinline__ void rb_load(RbBuf<T, NR> &b, const bool (&act)[NR],
         }
 }
 
-// acc += the hop from buffered operands; sg: antiperiodic sign (t hops only)
+// acc += the hop from the buffered spinors and the link u; sg: antiperiodic sign (t hops only)
 template <int MU, int SIGN, bool FWD, int NR, typename T>
-__device__ __forceinline__ void rb_compute(cx<typename St<T>::R> (&acc)[NR][4][3], const RbBuf<T, NR> &b,
-                                           const bool (&act)[NR], typename St<T>::R sg) {
+__device__ __forceinline__ void rb_compute_u(cx<typename St<T>::R> (&acc)[NR][4][3], const RbBuf<T, NR> &b,
+                                             const cx<typename St<T>::R> (&u)[3][3], const bool (&act)[NR],
+                                             typename St<T>::R sg) {
     using R = typename St<T>::R;
-    cx<R> u[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) u[i][j] = widen(b.u[i * 3 + j]);
 #pragma unroll
     for (int r = 0; r < NR; ++r)
         if (act[r]) {
@@ -175,6 +171,19 @@ __device__ __forceinline__ void rb_compute(cx<typename St<T>::R> (&acc)[NR][4][3
         }
 }
 
+// acc += the hop from buffered operands (spinors and link)
+template <int MU, int SIGN, bool FWD, int NR, typename T>
+__device__ __forceinline__ void rb_compute(cx<typename St<T>::R> (&acc)[NR][4][3], const RbBuf<T, NR> &b,
+                                           const bool (&act)[NR], typename St<T>::R sg) {
+    using R = typename St<T>::R;
+    cx<R> u[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) u[i][j] = widen(b.u[i * 3 + j]);
+    rb_compute_u<MU, SIGN, FWD, NR, T>(acc, b, u, act, sg);
+}
+
 // Hop H = 2 mu + (0 forward, 1 backward): issue hop H+1's loads, then compute hop H.
 template <typename T, int SIGN, int H, int NR, bool NTL>
 __device__ __forceinline__ void rb_chain(cx<typename St<T>::R> (&acc)[NR][4][3], RbBuf<T, NR> (&buf)[2],
@@ -193,6 +202,79 @@ __device__ __forceinline__ void rb_chain(cx<typename St<T>::R> (&acc)[NR][4][3],
         }
         rb_compute<H / 2, SIGN, (H % 2 == 0), NR, T>(acc, buf[H & 1], act, (H % 2 == 0) ? sf : sb);
         rb_chain<T, SIGN, H + 1, NR, NTL>(acc, buf, act, in, rs, gout, gin, S, k, n, sf, sb);
+    }
+}
+
+// Epilogue for the NR rhs of this lane at output site k: see Epi in wl_internal.h.  Norms of
+// the stored values are added to nrm.
+template <typename T, int EPI, int NR>
+__device__ __forceinline__ void rb_epilogue(const RbArgs<T> &A, cx<typename St<T>::R> (&acc)[NR][4][3],
+                                            const bool (&act)[NR], int jf, int k, double (&nrm)[NR]) {
+    using R = typename St<T>::R;
+    const int S = A.g.stride;
+    const size_t rs = (size_t)St<T>::planes * S;
+    const R a = A.a;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        if (!act[r]) continue;
+        const size_t eb = (size_t)(jf + r) * rs;
+        if constexpr (EPI == EPI_SET) {
+            cx<R> v[4][3];
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) v[s][c] = a * acc[r][s][c];
+            st_site<T, true>(A.out + eb, S, k, v);
+        } else if constexpr (EPI == EPI_AXPY || EPI == EPI_AXPY_NORM) {
+            cx<R> v[4][3];
+            ld_site<T, true>(A.xin + eb, S, k, v);
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) v[s][c] = v[s][c] + a * acc[r][s][c];
+            st_site<T, true>(A.out + eb, S, k, v);
+            if constexpr (EPI == EPI_AXPY_NORM) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) nrm[r] += (double)norm2(v[s][c]);
+            }
+        } else {  // EPI_CGUPD: x += alpha p (x: FP64 solution or FP32 inner y), r -= alpha DDp
+            const double alpha = A.st[jf + r].alpha;
+            const R alpha_r = (R)alpha;
+            cx<R> ddp[4][3], pv[4][3], rv[4][3];
+            ld_site<T, true>(A.xin + eb, S, k, ddp);
+            ld_site<T, true>(A.p + eb, S, k, pv);
+            ld_site<T, true>(A.r + eb, S, k, rv);
+            const size_t xb = (size_t)(jf + r) * 12 * S + k;
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const size_t e = xb + (size_t)(s * 3 + c) * S;
+                    stc<true>(A.x + e, ldc<true>(A.x + e) + alpha_r * pv[s][c]);
+                    const cx<R> dd = ddp[s][c] + a * acc[r][s][c];
+                    rv[s][c] = rv[s][c] - alpha_r * dd;
+                }
+            st_site<T, true>(A.r + eb, S, k, rv);
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) nrm[r] += (double)norm2(rv[s][c]);
+        }
+    }
+}
+
+// Per-wave partial sums of the norm epilogues: partials[rhs * pstride + segment o].
+template <typename T, int EPI, int NR>
+__device__ __forceinline__ void rb_partials(const RbArgs<T> &A, const double (&nrm)[NR], int jf, int o) {
+    if constexpr (EPI == EPI_AXPY_NORM || EPI == EPI_CGUPD) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const double s = wave_sum(nrm[r]);
+            if ((threadIdx.x & 63) == 0 && jf + r < A.jend && o < A.pstride)
+                A.partials[(size_t)(jf + r) * A.pstride + o] = s;
+        }
     }
 }
 
@@ -245,65 +327,103 @@ __global__ __launch_bounds__(kBlock, 2) void k_dslash_rb(RbArgs<T> A) {
             rb_hop<3, SIGN, false, NR, NTL, T>(acc, act, in, rs, gin + 27 * S, S, n.bwd[3], n.bwd[3], sb);
         }
 
-        const R a = A.a;
+        rb_epilogue<T, EPI, NR>(A, acc, act, jf, k, nrm);
+    }
+    rb_partials<T, EPI, NR>(A, nrm, jf, o);
+}
+
+// ---------------------------------------------------------- LDS-staged links ---------------
+// One workgroup per 64-site segment; its waves are the segment's rhs chunks (C <= 8).  The
+// segment's links (8 hops x 9 entries x 64 sites) are read from HBM once, nontemporally, into
+// LDS, shared by the C waves, so that link traffic is once per segment whatever the caches do;
+// each wave then streams only its NR rhs' neighbour spinors (pipelined by one hop when PIPE).
+
+template <int NR, typename T>
+__device__ __forceinline__ void sp_load(RbBuf<T, NR> &b, const bool (&act)[NR],
+                                        const typename St<T>::E *__restrict__ in, size_t rs, int S, int kn) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            const size_t eb = (size_t)(jf + r) * rs;
-            if constexpr (EPI == EPI_SET) {
-                cx<R> v[4][3];
+    for (int r = 0; r < NR; ++r)
+        if (act[r]) {
 #pragma unroll
-                for (int s = 0; s < 4; ++s)
+            for (int c = 0; c < 12; ++c) b.w[r][c] = ldraw<false>(in + r * rs + c * S + kn);
+            if constexpr (St<T>::scaled) b.sc[r] = scale_plane(in + r * rs, S)[kn];
+        }
+}
+
+template <typename T, int SIGN, int H, int NR, bool PIPE>
+__device__ __forceinline__ void rb_chain_lds(cx<typename St<T>::R> (&acc)[NR][4][3], RbBuf<T, NR> (&buf)[2],
+                                             const bool (&act)[NR], const typename St<T>::E *__restrict__ in,
+                                             size_t rs, const typename St<T>::E *lu, int lane, int S,
+                                             const SiteNb &n, typename St<T>::R sf, typename St<T>::R sb) {
+    using R = typename St<T>::R;
+    if constexpr (H < 8) {
+        if constexpr (PIPE) {
+            constexpr int HN = H + 1;
+            if constexpr (HN < 8)
+                sp_load<NR, T>(buf[HN & 1], act, in, rs, S, (HN % 2 == 0) ? n.fwd[HN / 2] : n.bwd[HN / 2]);
+        } else {
+            sp_load<NR, T>(buf[H & 1], act, in, rs, S, (H % 2 == 0) ? n.fwd[H / 2] : n.bwd[H / 2]);
+        }
+        cx<R> u[3][3];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) v[s][c] = a * acc[r][s][c];
-                st_site<T, true>(A.out + eb, S, k, v);
-            } else if constexpr (EPI == EPI_AXPY || EPI == EPI_AXPY_NORM) {
-                cx<R> v[4][3];
-                ld_site<T, true>(A.xin + eb, S, k, v);
+        for (int e = 0; e < 9; ++e) u[e / 3][e % 3] = widen(lu[(H * 9 + e) * 64 + lane]);
+        rb_compute_u<H / 2, SIGN, (H % 2 == 0), NR, T>(acc, buf[H & 1], u, act, (H % 2 == 0) ? sf : sb);
+        rb_chain_lds<T, SIGN, H + 1, NR, PIPE>(acc, buf, act, in, rs, lu, lane, S, n, sf, sb);
+    }
+}
+
+template <typename T, int SIGN, int EPI, int NR, bool PIPE>
+__global__ __launch_bounds__(512, 1) void k_dslash_rb_lds(RbArgs<T> A) {
+    using R = typename St<T>::R;
+    using E = typename St<T>::E;
+    extern __shared__ unsigned char rb_smem[];
+    E *lu = reinterpret_cast<E *>(rb_smem);  // [hop * 9 + entry][lane]
+    if (A.all_done && *A.all_done) return;   // grid-uniform
+    const int o = xcd_tile(blockIdx.x, gridDim.x);
+    const int chunk = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63), nw = (int)(blockDim.x >> 6);
+    const int k = segment_base(A.g, o) + lane;
+    const int jf = A.j0 + chunk * NR;
+    const int S = A.g.stride;
+    const size_t rs = (size_t)St<T>::planes * S;
+    bool act[NR];  // wave-uniform
 #pragma unroll
-                for (int s = 0; s < 4; ++s)
+    for (int r = 0; r < NR; ++r) {
+        const int j = jf + r;
+        act[r] = j < A.jend && (A.st == nullptr || A.st[j].done == 0);
+    }
+    double nrm[NR];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) v[s][c] = v[s][c] + a * acc[r][s][c];
-                st_site<T, true>(A.out + eb, S, k, v);
-                if constexpr (EPI == EPI_AXPY_NORM) {
-#pragma unroll
-                    for (int s = 0; s < 4; ++s)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) nrm[r] += (double)norm2(v[s][c]);
-                }
-            } else {  // EPI_CGUPD: x += alpha p (x: FP64 solution or FP32 inner y), r -= alpha DDp
-                const double alpha = A.st[jf + r].alpha;
-                const R alpha_r = (R)alpha;
-                cx<R> ddp[4][3], pv[4][3], rv[4][3];
-                ld_site<T, true>(A.xin + eb, S, k, ddp);
-                ld_site<T, true>(A.p + eb, S, k, pv);
-                ld_site<T, true>(A.r + eb, S, k, rv);
-                const size_t xb = (size_t)(jf + r) * 12 * S + k;
-#pragma unroll
-                for (int s = 0; s < 4; ++s)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const size_t e = xb + (size_t)(s * 3 + c) * S;
-                        stc<true>(A.x + e, ldc<true>(A.x + e) + alpha_r * pv[s][c]);
-                        const cx<R> dd = ddp[s][c] + a * acc[r][s][c];
-                        rv[s][c] = rv[s][c] - alpha_r * dd;
-                    }
-                st_site<T, true>(A.r + eb, S, k, rv);
-#pragma unroll
-                for (int s = 0; s < 4; ++s)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) nrm[r] += (double)norm2(rv[s][c]);
-            }
+    for (int r = 0; r < NR; ++r) nrm[r] = 0;
+    const bool live = k < A.g.vh;
+    SiteNb n{};
+    if (live) n = neighbours(A.g, k, A.out_par);
+    const E *gout = A.gauge + (size_t)A.out_par * 36 * S;
+    const E *gin = A.gauge + (size_t)(1 - A.out_par) * 36 * S;
+    if (live) {
+        for (int idx = chunk; idx < 72; idx += nw) {
+            const int h = idx / 9, e = idx - 9 * h, mu = h >> 1;
+            const E *src = (h & 1) ? gin + (size_t)(mu * 9 + e) * S + n.bwd[mu] : gout + (size_t)(mu * 9 + e) * S + k;
+            lu[idx * 64 + lane] = ldraw<true>(src);
         }
     }
-    if constexpr (EPI == EPI_AXPY_NORM || EPI == EPI_CGUPD) {
+    __syncthreads();
+    if (live) {
+        const R sf = (A.g.antiperiodic && n.fwd_edge_t) ? R(-1) : R(1);
+        const R sb = (A.g.antiperiodic && n.bwd_edge_t) ? R(-1) : R(1);
+        const E *in = A.in + jf * rs;
+        cx<R> acc[NR][4][3];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const double s = wave_sum(nrm[r]);
-            if ((threadIdx.x & 63) == 0 && jf + r < A.jend && o < A.pstride)
-                A.partials[(size_t)(jf + r) * A.pstride + o] = s;
-        }
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[r][s][c] = {R(0), R(0)};
+        RbBuf<T, NR> buf[2];
+        if constexpr (PIPE) sp_load<NR, T>(buf[0], act, in, rs, S, n.fwd[0]);
+        rb_chain_lds<T, SIGN, 0, NR, PIPE>(acc, buf, act, in, rs, lu, lane, S, n, sf, sb);
+        rb_epilogue<T, EPI, NR>(A, acc, act, jf, k, nrm);
     }
+    rb_partials<T, EPI, NR>(A, nrm, jf, o);
 }
 
 template <typename T, int SIGN, int EPI, int NR, bool NTL, bool PIPE>
@@ -342,8 +462,48 @@ inline hipError_t rb_pipe(const Geom &g, const MrLaunch &L, hipStream_t s) {
     return rb_launch<T, SIGN, EPI, NR, NTL, false>(g, L, s);
 }
 
+// Spinor pipelining in the LDS-staged kernel (no link registers): FP32 arithmetic up to 4 per
+// lane, FP64 up to 1.
+template <typename T, int NR>
+constexpr bool rb_lds_pipe_ok() {
+    return sizeof(typename St<T>::R) == 4 ? NR <= 4 : NR <= 1;
+}
+
+template <typename T, int SIGN, int EPI, int NR, bool PIPE>
+inline hipError_t rb_launch_lds(const Geom &g, const MrLaunch &L, hipStream_t s) {
+    using E = typename St<T>::E;
+    using R = typename St<T>::R;
+    RbArgs<T> A;
+    A.out = (E *)L.out;
+    A.in = (const E *)L.in;
+    A.xin = (const E *)L.xin;
+    A.gauge = (const E *)L.gauge;
+    A.a = (R)L.a;
+    A.g = g;
+    A.out_par = L.out_par;
+    A.j0 = L.j0;
+    A.jend = L.j0 + L.NB;
+    A.C = (L.NB + NR - 1) / NR;
+    A.all_done = L.all_done;
+    A.st = L.st;
+    A.x = (cx<R> *)L.x;
+    A.p = (const E *)L.p;
+    A.r = (E *)L.r;
+    A.partials = L.partials;
+    A.pstride = mr_segments(g);
+    const size_t lds = (size_t)72 * 64 * sizeof(E) + (size_t)g.lds;
+    hipLaunchKernelGGL((k_dslash_rb_lds<T, SIGN, EPI, NR, PIPE>), dim3((unsigned)A.pstride), dim3(64 * A.C),
+                       lds, s, A);
+    return hipGetLastError();
+}
+
 template <typename T, int SIGN, int EPI, int NR>
 inline hipError_t rb_ntl(const Geom &g, const MrLaunch &L, hipStream_t s) {
+    if (L.lds_links && (L.NB + NR - 1) / NR <= 8) {
+        if constexpr (rb_lds_pipe_ok<T, NR>())
+            if (L.pipe) return rb_launch_lds<T, SIGN, EPI, NR, true>(g, L, s);
+        return rb_launch_lds<T, SIGN, EPI, NR, false>(g, L, s);
+    }
     return L.link_nt ? rb_pipe<T, SIGN, EPI, NR, true>(g, L, s) : rb_pipe<T, SIGN, EPI, NR, false>(g, L, s);
 }
 

@@ -170,7 +170,8 @@ int wl_cg_solve_mixed(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa,
  *       every t before the next chunk (n must divide the segments per slice); -1 = default.
  *   WL_OPT_BLOCK  workgroup size 64, 128 or 256; 0 = default for the volume.
  *   WL_OPT_RHS_BLOCK  right-hand sides per lane of the multi-rhs kernel (register blocking,
- *       1..4; 0 = default: 4 in FP32, 2 in FP64).  A lane loads each link once for them.
+ *       1..4, at most 2 in FP64; 0 = default: 4 for FP32 storage, 2 for FP64 and FP16).  A
+ *       lane loads each link once for them.
  *   WL_OPT_RHS_LAUNCH  right-hand sides per launch, i.e. per pass over the lattice (rounded
  *       to a multiple of the per-lane count; 0 = default).  Fewer per pass keep the neighbour
  *       spinors of a pass in cache; each pass reads the links again.
@@ -179,7 +180,9 @@ int wl_cg_solve_mixed(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa,
  *       power-of-two distances and alias in the caches.  Only before the gauge field or any
  *       field of the context exists (else WL_E_STATE).
  *   WL_OPT_LINK_LOAD  link loads of the multi-rhs kernel: 0 = automatic (nontemporal when a
- *       launch reads each link once), 1 = cached, 2 = nontemporal.
+ *       launch reads each link once), 1 = cached, 2 = nontemporal, 3 = staged in LDS: one
+ *       workgroup per 64-site segment whose waves (at most 8) are the rhs chunks share one read
+ *       of the segment's links.
  *   WL_OPT_LDS_RESERVE  bytes of LDS reserved by every hopping workgroup (0..65536; 0 = none).
  *       The kernels use no LDS; the reservation caps how many workgroups share a CU, i.e. the
  *       sites in flight, whose neighbour spinors must fit the caches to be reused.

@@ -174,3 +174,38 @@ def test_graph_replayed_chunks_equal_eager_launches(precision):
             assert st[j]["iterations"] == st_eager[j]["iterations"]
             assert st[j]["true_resid"] == st_eager[j]["true_resid"]
     lat.set_option(W.OPT_GRAPHS, 0)
+
+
+@pytest.mark.parametrize("prec", [W.F64, W.F32, W.F16], ids=["f64", "f32", "f16"])
+def test_link_load_modes_give_the_same_operator(prec):
+    """Cached, nontemporal and LDS-staged link loads of the multi-rhs kernel, with and without
+    software pipelining, compute the same hopping term (to rounding: FMA contraction may differ
+    between instantiations); FP64 also against the oracle."""
+    dims, N = (8, 8, 8, 16), 6
+    lat, U = _setup(dims)
+    D = W.Dirac_Wilson_evenodd(0.125, lat)
+    srcs = [W.gen_spinor(dims, 700 + j, EVEN) for j in range(N)]
+    fin = W.Field(lat, EVEN, prec, nrhs=N)
+    for j, sj in enumerate(srcs):
+        fin.upload(sj, rhs=j)
+    fo = W.Field(lat, ODD, prec, nrhs=N)
+    tol = {W.F64: 1e-14, W.F32: 1e-6, W.F16: 1e-4}[prec]  # FP16: a rounding step may flip a half
+    ref = None
+    for nr in (1, 2):
+        for mode in (1, 2, 3):
+            for pipe in (0, 1):
+                lat.set_option(W.OPT_RHS_BLOCK, nr)
+                lat.set_option(W.OPT_LINK_LOAD, mode)
+                lat.set_option(W.OPT_PIPELINE, pipe)
+                D.hop_set(fin, fo, -1)
+                out = [fo.download(j) for j in range(N)]
+                if ref is None:
+                    ref = out
+                for j in range(N):
+                    assert rel(out[j], ref[j]) <= tol, (nr, mode, pipe, j)
+    lat.set_option(W.OPT_LINK_LOAD, 0)
+    lat.set_option(W.OPT_PIPELINE, 0)
+    if prec == W.F64:
+        orc = Oracle(dims)
+        for j in (0, N - 1):
+            assert rel(fo.to_full(j), orc.hop_set(U, 0.125, srcs[j], ODD, -1)) <= 1e-13
