@@ -39,9 +39,9 @@ def rel_diff(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def solve_series(W, D, fin, fx, n, precision, accuracy=1e-10):
+def solve_series(W, D, fin, fx, n, precision, accuracy=1e-10, delta=1e-3):
     """n solves of M^dag M x = b from x = 0; latency percentiles, iterations, true residuals."""
-    cg = W.CG(D, accuracy, 10000, precision=precision)
+    cg = W.CG(D, accuracy, 10000, precision=precision, delta=delta)
     lat_ms, iters, tres, upd = [], [], [], []
     for i in range(n):
         fx.zero()
@@ -74,8 +74,9 @@ def solve_series(W, D, fin, fx, n, precision, accuracy=1e-10):
 
 def mixed_precision_config(W, n):
     """Mixed-precision configuration: 48^3x96, links exp(i 0.3 H) (seed 6), kappa 0.1245,
-    Gaussian source (seed 7), CG to 1e-10 true residual in FP64 and with the FP32 inner solver;
-    the mixed solution's error is stated against the FP64 CG solution."""
+    Gaussian source (seed 7), CG to 1e-10 true residual in FP64, with the FP32 inner solver and
+    with the FP16 (site-scaled) inner solver; each mixed solution's error is stated against the
+    FP64 CG solution."""
     dims, kappa = (48, 48, 48, 96), 0.1245
     t0 = time.perf_counter()
     U = W.gen_gauge(dims, "smooth", 0.3, 6)
@@ -90,10 +91,14 @@ def mixed_precision_config(W, n):
     us64 = 1e3 * D.time(b, fo64, "hop", 1, warmup=3, iters=20)
     us32 = 1e3 * D.time(fin32, fo32, "hop", 1, warmup=3, iters=20)
     del fo64, fo32, fin32
-    x64, x32 = W.Field(lat), W.Field(lat)
+    x64, x32, x16 = W.Field(lat), W.Field(lat), W.Field(lat)
     r64 = solve_series(W, D, b, x64, n, "f64")
     r32 = solve_series(W, D, b, x32, n, "f32")
+    r16 = solve_series(W, D, b, x16, n, "f16", delta=1e-2)
+    r16["speedup_vs_f64"] = round(r16["solves_per_s"] / r64["solves_per_s"], 3)
+    r16["solution_rel_diff_vs_f64"] = rel_diff(x16.download(), x64.download())
     out = {"lattice": list(dims), "kappa": kappa, "f64": r64, "f32_mixed": r32,
+           "f16_mixed": r16,
            "speedup_vs_f64": round(r32["solves_per_s"] / r64["solves_per_s"], 3),
            "solution_rel_diff_vs_f64": rel_diff(x32.download(), x64.download()),
            "dslash_f64_us": round(us64, 1),

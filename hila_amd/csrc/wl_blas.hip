@@ -70,16 +70,20 @@ __global__ __launch_bounds__(kBlock) void k_convert(const cx<double> *__restrict
     out[i] = {(R)v.x, (R)v.y};
 }
 
-// FP64 links -> FP16 planes (SU(3) entries are bounded by 1: no scale)
+// FP64 links -> FP16 (SU(3) entries are bounded by 1: no scale), in the packed FP16 link
+// order of wl_store.h: element i = (block * 9 + e) * S + k of the FP64 planes is entry e of
+// site k of block (parity, direction).
 __global__ __launch_bounds__(kBlock) void k_convert_h2(const cx<double> *__restrict__ in,
-                                                       h2 *__restrict__ out, size_t n) {
+                                                       h2 *__restrict__ out, size_t n, int S) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
+    const size_t blk = i / ((size_t)9 * S), rem = i - blk * 9 * S;
+    const int e = (int)(rem / S), k = (int)(rem - (size_t)e * S);
     const cx<double> v = in[i];
     h2 q;
     q.x = (_Float16)(float)v.x;
     q.y = (_Float16)(float)v.y;
-    out[i] = q;
+    out[blk * 9 * S + link_entry<f16s>(e, S, k)] = q;
 }
 
 // ------------------------------------------------------------------ reductions ---------
@@ -251,7 +255,7 @@ hipError_t launch_gauge_convert(const Geom &g, int prec, const void *gauge64, vo
                            (const cx<double> *)gauge64, (cx<float> *)gauge_lo, n);
     else if (prec == WL_F16)
         hipLaunchKernelGGL(k_convert_h2, grid, dim3(kBlock), 0, s, (const cx<double> *)gauge64,
-                           (h2 *)gauge_lo, n);
+                           (h2 *)gauge_lo, n, g.stride);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

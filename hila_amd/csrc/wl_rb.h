@@ -62,19 +62,21 @@ __device__ __forceinline__ void rb_hop(cx<typename St<T>::R> (&acc)[NR][4][3], c
                                        int kl, typename St<T>::R sg) {
     using R = typename St<T>::R;
     cx<R> u[3][3];
+    {
+        typename St<T>::E ur[9];
+        ld_link_raw<NTL, T>(ur, link, S, kl);
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) u[i][j] = lde<NTL>(link + (i * 3 + j) * S + kl);
+        for (int e = 0; e < 9; ++e) u[e / 3][e % 3] = widen(ur[e]);
+    }
     cx<R> w[NR][4][3];
     R sc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r)
         if (act[r]) {
+            typename St<T>::E wr[12];
+            ld_spinor_raw<false, T>(wr, in + r * rs, S, kn);
 #pragma unroll
-            for (int s = 0; s < 4; ++s)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) w[r][s][c] = lde<false>(in + r * rs + (s * 3 + c) * S + kn);
+            for (int c = 0; c < 12; ++c) w[r][c / 3][c % 3] = widen(wr[c]);
             if constexpr (St<T>::scaled)
                 sc[r] = sg * scale_plane(in + r * rs, S)[kn];
             else
@@ -126,13 +128,11 @@ __device__ __forceinline__ void rb_load(RbBuf<T, NR> &b, const bool (&act)[NR],
                                         const typename St<T>::E *__restrict__ in, size_t rs,
                                         const typename St<T>::E *__restrict__ link, int S, int kn,
                                         int kl) {
-#pragma unroll
-    for (int e = 0; e < 9; ++e) b.u[e] = ldraw<NTL>(link + e * S + kl);
+    ld_link_raw<NTL, T>(b.u, link, S, kl);
 #pragma unroll
     for (int r = 0; r < NR; ++r)
         if (act[r]) {
-#pragma unroll
-            for (int c = 0; c < 12; ++c) b.w[r][c] = ldraw<false>(in + r * rs + c * S + kn);
+            ld_spinor_raw<false, T>(b.w[r], in + r * rs, S, kn);
             if constexpr (St<T>::scaled) b.sc[r] = scale_plane(in + r * rs, S)[kn];
         }
 }
@@ -344,8 +344,7 @@ __device__ __forceinline__ void sp_load(RbBuf<T, NR> &b, const bool (&act)[NR],
 #pragma unroll
     for (int r = 0; r < NR; ++r)
         if (act[r]) {
-#pragma unroll
-            for (int c = 0; c < 12; ++c) b.w[r][c] = ldraw<false>(in + r * rs + c * S + kn);
+            ld_spinor_raw<false, T>(b.w[r], in + r * rs, S, kn);
             if constexpr (St<T>::scaled) b.sc[r] = scale_plane(in + r * rs, S)[kn];
         }
 }
@@ -402,7 +401,8 @@ __global__ __launch_bounds__(512, 1) void k_dslash_rb_lds(RbArgs<T> A) {
     if (live) {
         for (int idx = chunk; idx < 72; idx += nw) {
             const int h = idx / 9, e = idx - 9 * h, mu = h >> 1;
-            const E *src = (h & 1) ? gin + (size_t)(mu * 9 + e) * S + n.bwd[mu] : gout + (size_t)(mu * 9 + e) * S + k;
+            const E *src = (h & 1) ? gin + (size_t)mu * 9 * S + link_entry<T>(e, S, n.bwd[mu])
+                                   : gout + (size_t)mu * 9 * S + link_entry<T>(e, S, k);
             lu[idx * 64 + lane] = ldraw<true>(src);
         }
     }

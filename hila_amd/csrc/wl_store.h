@@ -3,13 +3,18 @@
 //
 //   double  FP64 complex planes, FP64 arithmetic
 //   float   FP32 complex planes, FP32 arithmetic
-//   f16s    FP16 complex planes (re, im as two halves in 4 bytes) with one FP32 scale per site
+//   f16s    FP16 complex numbers (re, im as two halves in 4 bytes) with one FP32 scale per site
 //           and rhs, FP32 arithmetic.  A site's 24 reals are stored as v / m, m = the largest
 //           |re|, |im| of the site, so each stored half lies in [-1, 1] and the relative error
-//           of a component is at most 2^-11 of the site's largest one.  Links are stored as
-//           plain FP16 planes (SU(3) entries are bounded by 1).
-// A right-hand side of an N-rhs field is a block of `planes` planes of `stride` elements: the
-// 12 spin-colour component planes, and for f16s one more plane holding the FP32 site scales.
+//           of a component is at most 2^-11 of the site's largest one.  Links are stored
+//           without scale (SU(3) entries are bounded by 1).
+// A right-hand side of an N-rhs field is a block of `planes` x `stride` elements.  FP64 and
+// FP32: 12 spin-colour component planes, component c of site k at element c * S + k.  f16s:
+// the 12 components are packed 4 to a 16-byte vector, so that a lane reads a site's spinor
+// with 3 vector loads instead of 12 narrow ones: component c of site k at element
+// (c / 4) * 4S + 4k + c % 4, followed by the plane of FP32 site scales at element 12S (13S
+// elements of 4 bytes in all).  FP16 links likewise: entries 0-3 and 4-7 of site k as two
+// vectors, at (e / 4) * 4S + 4k + e % 4, entry 8 at 8S + k.
 #pragma once
 
 #include "wl_device.h"
@@ -89,6 +94,66 @@ __device__ __forceinline__ cx<double> widen(cx<double> v) { return v; }
 __device__ __forceinline__ cx<float> widen(cx<float> v) { return v; }
 __device__ __forceinline__ cx<float> widen(h2 v) { return {(float)v.x, (float)v.y}; }
 
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+struct h2x4 {
+    h2 e[4];
+};
+
+// Raw (packed) loads of a site's 12 spinor components and 9 link entries.
+template <bool NT, typename T>
+__device__ __forceinline__ void ld_spinor_raw(typename St<T>::E (&w)[12], const typename St<T>::E *base, int S,
+                                              int k) {
+    if constexpr (St<T>::scaled) {
+        const u32x4 *vb = reinterpret_cast<const u32x4 *>(base);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            u32x4 v;
+            if constexpr (NT)
+                v = __builtin_nontemporal_load(vb + (size_t)q * S + k);
+            else
+                v = vb[(size_t)q * S + k];
+            const h2x4 h = __builtin_bit_cast(h2x4, v);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) w[4 * q + i] = h.e[i];
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 12; ++c) w[c] = ldraw<NT>(base + (size_t)c * S + k);
+    }
+}
+
+template <bool NT, typename T>
+__device__ __forceinline__ void ld_link_raw(typename St<T>::E (&u)[9], const typename St<T>::E *base, int S,
+                                            int k) {
+    if constexpr (St<T>::scaled) {
+        const u32x4 *vb = reinterpret_cast<const u32x4 *>(base);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            u32x4 v;
+            if constexpr (NT)
+                v = __builtin_nontemporal_load(vb + (size_t)q * S + k);
+            else
+                v = vb[(size_t)q * S + k];
+            const h2x4 h = __builtin_bit_cast(h2x4, v);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) u[4 * q + i] = h.e[i];
+        }
+        u[8] = ldraw<NT>(base + (size_t)8 * S + k);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) u[e] = ldraw<NT>(base + (size_t)e * S + k);
+    }
+}
+
+// Element of link entry e of site k in a (parity, direction) block of 9 x S elements.
+template <typename T>
+__device__ __forceinline__ size_t link_entry(int e, int S, int k) {
+    if constexpr (St<T>::scaled)
+        return e < 8 ? (size_t)(e >> 2) * 4 * S + 4 * (size_t)k + (e & 3) : (size_t)8 * S + k;
+    else
+        return (size_t)e * S + k;
+}
+
 // FP32 scale plane of the rhs block starting at `base` (f16s only).
 __device__ __forceinline__ const float *scale_plane(const h2 *base, int S) {
     return reinterpret_cast<const float *>(base + (size_t)12 * S);
@@ -102,10 +167,10 @@ __device__ __forceinline__ float *scale_plane(h2 *base, int S) {
 template <typename T, bool NT = false>
 __device__ __forceinline__ void ld_site(const typename St<T>::E *base, int S, int k,
                                         cx<typename St<T>::R> (&w)[4][3]) {
+    typename St<T>::E raw[12];
+    ld_spinor_raw<NT, T>(raw, base, S, k);
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) w[s][c] = lde<NT>(base + (s * 3 + c) * S + k);
+    for (int c = 0; c < 12; ++c) w[c / 3][c % 3] = widen(raw[c]);
     if constexpr (St<T>::scaled) {
         const float m = scale_plane(base, S)[k];
 #pragma unroll
@@ -132,20 +197,25 @@ __device__ __forceinline__ void st_site(typename St<T>::E *base, int S, int k,
 #pragma unroll
             for (int c = 0; c < 3; ++c) m = fmaxf(m, fmaxf(fabsf(v[s][c].x), fabsf(v[s][c].y)));
         const float inv = m > 0.f ? 1.0f / m : 0.f;
+        u32x4 *vb = reinterpret_cast<u32x4 *>(base);
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+        for (int q4 = 0; q4 < 3; ++q4) {
+            h2x4 h;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
+            for (int i = 0; i < 4; ++i) {
+                const int c = 4 * q4 + i;
                 h2 q;
-                q.x = (_Float16)(v[s][c].x * inv);
-                q.y = (_Float16)(v[s][c].y * inv);
-                const unsigned bits = __builtin_bit_cast(unsigned, q);
-                if constexpr (NT)
-                    __builtin_nontemporal_store(bits, (unsigned *)(base + (s * 3 + c) * S + k));
-                else
-                    *(unsigned *)(base + (s * 3 + c) * S + k) = bits;
-                v[s][c] = {m * (float)q.x, m * (float)q.y};
+                q.x = (_Float16)(v[c / 3][c % 3].x * inv);
+                q.y = (_Float16)(v[c / 3][c % 3].y * inv);
+                h.e[i] = q;
+                v[c / 3][c % 3] = {m * (float)q.x, m * (float)q.y};
             }
+            const u32x4 bits = __builtin_bit_cast(u32x4, h);
+            if constexpr (NT)
+                __builtin_nontemporal_store(bits, vb + (size_t)q4 * S + k);
+            else
+                vb[(size_t)q4 * S + k] = bits;
+        }
         if constexpr (NT)
             __builtin_nontemporal_store(m, scale_plane(base, S) + k);
         else
