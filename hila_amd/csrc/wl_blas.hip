@@ -61,13 +61,16 @@ __global__ __launch_bounds__(kBlock) void k_gauge_to_soa(const double *__restric
     for (int e = 0; e < 9; ++e) dst[e * S + k] = {src[2 * e], src[2 * e + 1]};
 }
 
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_convert(const cx<double> *__restrict__ in,
-                                                    cx<R> *__restrict__ out, size_t n) {
+// FP64 links -> FP32, in the packed FP32 link order of wl_store.h: element i = (block * 9 + e)
+// * S + k of the FP64 planes is entry e of site k of block (parity, direction).
+__global__ __launch_bounds__(kBlock) void k_convert_f32(const cx<double> *__restrict__ in,
+                                                        cx<float> *__restrict__ out, size_t n, int S) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
+    const size_t blk = i / ((size_t)9 * S), rem = i - blk * 9 * S;
+    const int e = (int)(rem / S), k = (int)(rem - (size_t)e * S);
     const cx<double> v = in[i];
-    out[i] = {(R)v.x, (R)v.y};
+    out[blk * 9 * S + link_entry<float>(e, S, k)] = {(float)v.x, (float)v.y};
 }
 
 // FP64 links -> FP16 (SU(3) entries are bounded by 1: no scale), in the packed FP16 link
@@ -131,11 +134,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_xpay(cx<R> *__restrict__ p,
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= vh) return;
     const R beta = (R)st->beta;
+    cx<R> pv[4][3], rv[4][3];
+    ld_site<R>(p, S, k, pv);
+    ld_site<R>(r, S, k, rv);
 #pragma unroll
-    for (int c = 0; c < 12; ++c) {
-        const int e = c * S + k;
-        p[e] = beta * p[e] + r[e];
-    }
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pv[s][c] = beta * pv[s][c] + rv[s][c];
+    st_site<R>(p, S, k, pv);
 }
 
 // Reliable update: r = b - DDp in FP64 (DDp = M^dag M x in FP64), stored in the inner
@@ -148,13 +154,15 @@ __global__ __launch_bounds__(kBlock) void k_residual_update(const cx<double> *__
     const int k = blockIdx.x * kBlock + threadIdx.x;
     double s = 0;
     if (k < vh) {
+        cx<R> v[4][3];
 #pragma unroll
         for (int c = 0; c < 12; ++c) {
             const int e = c * S + k;
-            const cx<double> v = b[e] - ddp[e];
-            r[e] = {(R)v.x, (R)v.y};
-            s += norm2(v);
+            const cx<double> d = b[e] - ddp[e];
+            v[c / 3][c % 3] = {(R)d.x, (R)d.y};
+            s += norm2(d);
         }
+        st_site<R>(r, S, k, v);
     }
     reduce_partial(s, ra);
 }
@@ -251,8 +259,8 @@ hipError_t launch_gauge_convert(const Geom &g, int prec, const void *gauge64, vo
     const size_t n = gauge_elems(g);
     const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
     if (prec == WL_F32)
-        hipLaunchKernelGGL(k_convert<float>, grid, dim3(kBlock), 0, s,
-                           (const cx<double> *)gauge64, (cx<float> *)gauge_lo, n);
+        hipLaunchKernelGGL(k_convert_f32, grid, dim3(kBlock), 0, s, (const cx<double> *)gauge64,
+                           (cx<float> *)gauge_lo, n, g.stride);
     else if (prec == WL_F16)
         hipLaunchKernelGGL(k_convert_h2, grid, dim3(kBlock), 0, s, (const cx<double> *)gauge64,
                            (h2 *)gauge_lo, n, g.stride);

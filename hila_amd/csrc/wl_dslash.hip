@@ -21,6 +21,7 @@
 
 #include "wl_hop.h"
 #include "wl_internal.h"
+#include "wl_store.h"
 
 namespace wl {
 
@@ -53,36 +54,44 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
 
         const R a = A.a;
         if constexpr (EPI == EPI_SET) {
+            cx<R> v[4][3];
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) stc<true>(A.out + (s * 3 + c) * S + k, a * acc[s][c]);
+                for (int c = 0; c < 3; ++c) v[s][c] = a * acc[s][c];
+            st_site<R, true>(A.out, S, k, v);
         } else if constexpr (EPI == EPI_AXPY || EPI == EPI_AXPY_NORM) {
+            cx<R> v[4][3];
+            ld_site<R, true>(A.xin, S, k, v);
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const int e = (s * 3 + c) * S + k;
-                    const cx<R> o = ldc<true>(A.xin + e) + a * acc[s][c];
-                    stc<true>(A.out + e, o);
-                    if constexpr (EPI == EPI_AXPY_NORM) nrm += (double)norm2(o);
+                    v[s][c] = v[s][c] + a * acc[s][c];
+                    if constexpr (EPI == EPI_AXPY_NORM) nrm += (double)norm2(v[s][c]);
                 }
+            st_site<R, true>(A.out, S, k, v);
         } else {  // EPI_CGUPD: x (FP64) += alpha p, r -= alpha DDp in R
             const double alpha = A.red.st->alpha;
             const R alpha_r = (R)alpha;
+            cx<R> ddp[4][3], pv[4][3], rv[4][3];
+            ld_site<R, true>(A.xin, S, k, ddp);
+            ld_site<R, true>(A.p, S, k, pv);
+            ld_site<R, true>(A.r, S, k, rv);
+            cx<double> xv[4][3];
+            ld_site<double, true>(A.x, S, k, xv);
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const int e = (s * 3 + c) * S + k;
-                    const cx<R> ddp = ldc<true>(A.xin + e) + a * acc[s][c];
-                    const cx<R> pe = ldc<true>(A.p + e);
-                    const cx<double> pd = {(double)pe.x, (double)pe.y};
-                    stc<true>(A.x + e, ldc<true>(A.x + e) + alpha * pd);
-                    const cx<R> rn = ldc<true>(A.r + e) - alpha_r * ddp;
-                    stc<true>(A.r + e, rn);
-                    nrm += (double)norm2(rn);
+                    const cx<double> pd = {(double)pv[s][c].x, (double)pv[s][c].y};
+                    xv[s][c] = xv[s][c] + alpha * pd;
+                    const cx<R> dd = ddp[s][c] + a * acc[s][c];
+                    rv[s][c] = rv[s][c] - alpha_r * dd;
+                    nrm += (double)norm2(rv[s][c]);
                 }
+            st_site<double, true>(A.x, S, k, xv);
+            st_site<R, true>(A.r, S, k, rv);
         }
     }
     if constexpr (EPI == EPI_AXPY_NORM || EPI == EPI_CGUPD) reduce_partial(nrm, A.red);

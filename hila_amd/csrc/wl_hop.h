@@ -11,6 +11,7 @@
 #pragma once
 
 #include "wl_device.h"
+#include "wl_store.h"
 
 namespace wl {
 
@@ -21,19 +22,19 @@ struct HopBuf {
     cx<R> u[3][3];
 };
 
-// Spinor component c of the neighbour is at in[c * Ssp + kn]; link entry e at link[e * Slk + kl].
+// The neighbour spinor at site kn of the field `in` and the link entries at site kl of the link
+// block `link`, in the packed layouts of wl_store.h (FP64 and FP32 storage).
 template <bool NTL = true, typename R>
 __device__ __forceinline__ void hop_load(HopBuf<R> &b, const cx<R> *__restrict__ in,
                                          const cx<R> *__restrict__ link, int Ssp, int kn,
                                          int Slk, int kl) {
+    cx<R> w[12], u[9];
+    ld_spinor_raw<false, R>(w, in, Ssp, kn);
+    ld_link_raw<NTL, R>(u, link, Slk, kl);
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+    for (int c = 0; c < 12; ++c) b.w[c / 3][c % 3] = w[c];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) b.w[s][c] = in[(s * 3 + c) * Ssp + kn];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) b.u[i][j] = ldc<NTL>(link + (i * 3 + j) * Slk + kl);
+    for (int e = 0; e < 9; ++e) b.u[e / 3][e % 3] = u[e];
 }
 
 // acc += expand(U (1 + s g_mu) w) for a forward hop, expand(U^dag (1 - s g_mu) w) backward;

@@ -8,13 +8,12 @@
 //           |re|, |im| of the site, so each stored half lies in [-1, 1] and the relative error
 //           of a component is at most 2^-11 of the site's largest one.  Links are stored
 //           without scale (SU(3) entries are bounded by 1).
-// A right-hand side of an N-rhs field is a block of `planes` x `stride` elements.  FP64 and
-// FP32: 12 spin-colour component planes, component c of site k at element c * S + k.  f16s:
-// the 12 components are packed 4 to a 16-byte vector, so that a lane reads a site's spinor
-// with 3 vector loads instead of 12 narrow ones: component c of site k at element
-// (c / 4) * 4S + 4k + c % 4, followed by the plane of FP32 site scales at element 12S (13S
-// elements of 4 bytes in all).  FP16 links likewise: entries 0-3 and 4-7 of site k as two
-// vectors, at (e / 4) * 4S + 4k + e % 4, entry 8 at 8S + k.
+// A right-hand side of an N-rhs field is a block of `planes` x `stride` elements.  Its 12
+// spin-colour components are packed P to a 16-byte vector (P = 1 FP64, 2 FP32, 4 FP16), so
+// that a lane reads a site's spinor with 12 / P vector loads: component c of site k is element
+// (c / P) * PS + Pk + c % P.  f16s adds the plane of FP32 site scales at element 12S (13S
+// elements of 4 bytes in all).  Links likewise: entries 0-7 of site k packed, at
+// (e / P) * PS + Pk + e % P, entry 8 at 8S + k, in a (parity, direction) block of 9S.
 #pragma once
 
 #include "wl_device.h"
@@ -34,6 +33,7 @@ struct St<double> {
     using R = double;       // arithmetic
     using E = cx<double>;   // stored element
     static constexpr int planes = 12;
+    static constexpr int pack = 1;  // complex numbers per 16-byte vector
     static constexpr bool scaled = false;
 };
 template <>
@@ -41,6 +41,7 @@ struct St<float> {
     using R = float;
     using E = cx<float>;
     static constexpr int planes = 12;
+    static constexpr int pack = 2;
     static constexpr bool scaled = false;
 };
 template <>
@@ -48,6 +49,7 @@ struct St<f16s> {
     using R = float;
     using E = h2;
     static constexpr int planes = 13;
+    static constexpr int pack = 4;
     static constexpr bool scaled = true;
 };
 
@@ -95,63 +97,68 @@ __device__ __forceinline__ cx<float> widen(cx<float> v) { return v; }
 __device__ __forceinline__ cx<float> widen(h2 v) { return {(float)v.x, (float)v.y}; }
 
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-struct h2x4 {
-    h2 e[4];
+template <typename E, int P>
+struct Packed {
+    E e[P];
 };
+
+// One 16-byte vector of P packed elements at v (16-byte aligned).
+template <bool NT, typename T>
+__device__ __forceinline__ Packed<typename St<T>::E, St<T>::pack> ld_vec(const typename St<T>::E *v) {
+    const u32x4 *p = reinterpret_cast<const u32x4 *>(v);
+    u32x4 bits;
+    if constexpr (NT)
+        bits = __builtin_nontemporal_load(p);
+    else
+        bits = *p;
+    return __builtin_bit_cast(Packed<typename St<T>::E, St<T>::pack>, bits);
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_vec(typename St<T>::E *v, const Packed<typename St<T>::E, St<T>::pack> &x) {
+    const u32x4 bits = __builtin_bit_cast(u32x4, x);
+    u32x4 *p = reinterpret_cast<u32x4 *>(v);
+    if constexpr (NT)
+        __builtin_nontemporal_store(bits, p);
+    else
+        *p = bits;
+}
+
+// Element of spinor component c of site k in an rhs block; of link entry e in a link block.
+template <typename T>
+__device__ __forceinline__ size_t comp_elem(int c, int S, int k) {
+    constexpr int P = St<T>::pack;
+    return (size_t)(c / P) * P * S + (size_t)P * k + (c % P);
+}
+template <typename T>
+__device__ __forceinline__ size_t link_entry(int e, int S, int k) {
+    constexpr int P = St<T>::pack;
+    return e < 8 ? (size_t)(e / P) * P * S + (size_t)P * k + (e % P) : (size_t)8 * S + k;
+}
 
 // Raw (packed) loads of a site's 12 spinor components and 9 link entries.
 template <bool NT, typename T>
 __device__ __forceinline__ void ld_spinor_raw(typename St<T>::E (&w)[12], const typename St<T>::E *base, int S,
                                               int k) {
-    if constexpr (St<T>::scaled) {
-        const u32x4 *vb = reinterpret_cast<const u32x4 *>(base);
+    constexpr int P = St<T>::pack;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            u32x4 v;
-            if constexpr (NT)
-                v = __builtin_nontemporal_load(vb + (size_t)q * S + k);
-            else
-                v = vb[(size_t)q * S + k];
-            const h2x4 h = __builtin_bit_cast(h2x4, v);
+    for (int q = 0; q < 12 / P; ++q) {
+        const auto v = ld_vec<NT, T>(base + (size_t)q * P * S + (size_t)P * k);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) w[4 * q + i] = h.e[i];
-        }
-    } else {
-#pragma unroll
-        for (int c = 0; c < 12; ++c) w[c] = ldraw<NT>(base + (size_t)c * S + k);
+        for (int i = 0; i < P; ++i) w[P * q + i] = v.e[i];
     }
 }
 
 template <bool NT, typename T>
 __device__ __forceinline__ void ld_link_raw(typename St<T>::E (&u)[9], const typename St<T>::E *base, int S,
                                             int k) {
-    if constexpr (St<T>::scaled) {
-        const u32x4 *vb = reinterpret_cast<const u32x4 *>(base);
+    constexpr int P = St<T>::pack;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            u32x4 v;
-            if constexpr (NT)
-                v = __builtin_nontemporal_load(vb + (size_t)q * S + k);
-            else
-                v = vb[(size_t)q * S + k];
-            const h2x4 h = __builtin_bit_cast(h2x4, v);
+    for (int q = 0; q < 8 / P; ++q) {
+        const auto v = ld_vec<NT, T>(base + (size_t)q * P * S + (size_t)P * k);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) u[4 * q + i] = h.e[i];
-        }
-        u[8] = ldraw<NT>(base + (size_t)8 * S + k);
-    } else {
-#pragma unroll
-        for (int e = 0; e < 9; ++e) u[e] = ldraw<NT>(base + (size_t)e * S + k);
+        for (int i = 0; i < P; ++i) u[P * q + i] = v.e[i];
     }
-}
-
-// Element of link entry e of site k in a (parity, direction) block of 9 x S elements.
-template <typename T>
-__device__ __forceinline__ size_t link_entry(int e, int S, int k) {
-    if constexpr (St<T>::scaled)
-        return e < 8 ? (size_t)(e >> 2) * 4 * S + 4 * (size_t)k + (e & 3) : (size_t)8 * S + k;
-    else
-        return (size_t)e * S + k;
+    u[8] = ldraw<NT>(base + (size_t)8 * S + k);
 }
 
 // FP32 scale plane of the rhs block starting at `base` (f16s only).
@@ -185,11 +192,16 @@ __device__ __forceinline__ void ld_site(const typename St<T>::E *base, int S, in
 template <typename T, bool NT = false>
 __device__ __forceinline__ void st_site(typename St<T>::E *base, int S, int k,
                                         cx<typename St<T>::R> (&v)[4][3]) {
+    constexpr int P = St<T>::pack;
+    using E = typename St<T>::E;
     if constexpr (!St<T>::scaled) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+        for (int q = 0; q < 12 / P; ++q) {
+            Packed<E, P> x;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) stc<NT>(base + (s * 3 + c) * S + k, v[s][c]);
+            for (int i = 0; i < P; ++i) x.e[i] = v[(P * q + i) / 3][(P * q + i) % 3];
+            st_vec<NT, T>(base + (size_t)q * P * S + (size_t)P * k, x);
+        }
     } else {
         float m = 0.f;
 #pragma unroll
@@ -197,24 +209,19 @@ __device__ __forceinline__ void st_site(typename St<T>::E *base, int S, int k,
 #pragma unroll
             for (int c = 0; c < 3; ++c) m = fmaxf(m, fmaxf(fabsf(v[s][c].x), fabsf(v[s][c].y)));
         const float inv = m > 0.f ? 1.0f / m : 0.f;
-        u32x4 *vb = reinterpret_cast<u32x4 *>(base);
 #pragma unroll
-        for (int q4 = 0; q4 < 3; ++q4) {
-            h2x4 h;
+        for (int q = 0; q < 12 / P; ++q) {
+            Packed<E, P> x;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int c = 4 * q4 + i;
-                h2 q;
-                q.x = (_Float16)(v[c / 3][c % 3].x * inv);
-                q.y = (_Float16)(v[c / 3][c % 3].y * inv);
-                h.e[i] = q;
-                v[c / 3][c % 3] = {m * (float)q.x, m * (float)q.y};
+            for (int i = 0; i < P; ++i) {
+                const int c = P * q + i;
+                h2 h;
+                h.x = (_Float16)(v[c / 3][c % 3].x * inv);
+                h.y = (_Float16)(v[c / 3][c % 3].y * inv);
+                x.e[i] = h;
+                v[c / 3][c % 3] = {m * (float)h.x, m * (float)h.y};
             }
-            const u32x4 bits = __builtin_bit_cast(u32x4, h);
-            if constexpr (NT)
-                __builtin_nontemporal_store(bits, vb + (size_t)q4 * S + k);
-            else
-                vb[(size_t)q4 * S + k] = bits;
+            st_vec<NT, T>(base + (size_t)q * P * S + (size_t)P * k, x);
         }
         if constexpr (NT)
             __builtin_nontemporal_store(m, scale_plane(base, S) + k);
