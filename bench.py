@@ -432,6 +432,32 @@ def profile_serve(W):
           f"{wall:.2f} s", file=sys.stderr, flush=True)
 
 
+def config_io(W, lat, U, reps=3):
+    """HILA configuration file of the 32^4 gauge field (GaugeField::config_write / config_read,
+    604 MB) written and read back through the library: the file is streamed through pinned
+    chunks and reordered on the GPU.  Best of `reps`; the read follows the write, so the file
+    comes from the page cache.  Checks that the field read back is the uploaded one bit for bit."""
+    import tempfile
+    fd, path = tempfile.mkstemp(suffix=".cfg")
+    os.close(fd)
+    try:
+        tw, tr = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            lat.config_write(path)
+            tw.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            lat.config_read(path)
+            tr.append(time.perf_counter() - t0)
+        size = os.path.getsize(path)
+        exact = lat.download_gauge().tobytes() == U.tobytes()
+    finally:
+        os.remove(path)
+    return {"file_bytes": size, "write_ms": round(1e3 * min(tw), 1), "read_ms": round(1e3 * min(tr), 1),
+            "write_GBps": round(size / min(tw) / 1e9, 2), "read_GBps": round(size / min(tr) / 1e9, 2),
+            "read_back_bit_exact": exact, "file": "temporary file, read from the page cache"}
+
+
 def cpu_baseline(U, psi, seconds=15.0):
     """The oracle's HILA-structured hopping term (projection pass + per-direction
     accumulation, wilson.h:53-95) on the host cores, on a bounded number of applications."""
@@ -470,6 +496,8 @@ def main(argv=None):
     ap.add_argument("--no-multi", action="store_true",
                     help="skip the batched multi-rhs configuration")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving configuration")
+    ap.add_argument("--no-config-io", action="store_true",
+                    help="skip writing and reading the 32^4 configuration file")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--profile-multi", action="store_true",
                     help="short run for rocprofv3: --steps autotuned multi-rhs hopping launches "
@@ -546,6 +574,11 @@ def main(argv=None):
     print(f"# dslash {ms_kernel * 1e3:.1f} us/launch {gbs_kernel:.0f} GB/s", file=sys.stderr,
           flush=True)
 
+    cfg_io = None
+    if not a.no_config_io:
+        cfg_io = config_io(W, lat, U)
+        print(f"# config file: {cfg_io}", file=sys.stderr, flush=True)
+
     # --- CG solves to 1e-10: FP64 (HILA's algorithm) and mixed FP32 with reliable updates ---
     x64 = W.Field(lat, W.EVEN)
     cg_out = solve_series(W, D, fin, x64, a.cg_solves, "f64")
@@ -598,6 +631,7 @@ def main(argv=None):
         "mixed_48x48x48x96": big,
         "multi_rhs_32x32x32x32": multi,
         "serving": serve,
+        "config_file_32x32x32x32": cfg_io,
         "cpu_baseline": None,
     }
     if not a.no_cpu:

@@ -86,5 +86,7 @@ int field_upload_async(wl_ctx *c, void *data, int prec, int N, int j, const doub
 int field_download_async(wl_ctx *c, const void *data, int prec, int N, int j, double *host);
 // Gauge upload (U[mu]: V x 18 doubles), enqueued on the context's stream.
 int gauge_upload_async(wl_ctx *c, const double *const U[4]);
+// Gauge field from a HILA configuration file (wl_config.hip); returns after the upload.
+int config_upload(wl_ctx *c, const char *path);
 
 }  // namespace wl

@@ -46,6 +46,7 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 struct Request {
     uint64_t ticket = 0;
     wl_solve_request r{};
+    std::string config_path;  // owned copy of r.config_path
     Clock::time_point t_submit, t_admit, t_done;
     wl_solve_result res{};
     std::vector<wl_cg_stats> stats;
@@ -200,8 +201,12 @@ int create_slot(wl_server *S, const wl_solve_request &r, int cap,
 int load_slot(Slot *s, const wl_solve_request &r) {
     wl_ctx *c = s->ctx;
     int rc;
-    const double *U[4] = {r.gauge[0], r.gauge[1], r.gauge[2], r.gauge[3]};
-    if ((rc = gauge_upload_async(c, U))) return rc;
+    if (r.config_path) {
+        if ((rc = config_upload(c, r.config_path))) return rc;
+    } else {
+        const double *U[4] = {r.gauge[0], r.gauge[1], r.gauge[2], r.gauge[3]};
+        if ((rc = gauge_upload_async(c, U))) return rc;
+    }
     if ((rc = BatchCG::prepare(c, s->cap, r.inner_prec, r.inner_prec != WL_F64))) return rc;
     const size_t blk = (size_t)c->g.vh * 24;
     for (int j = 0; j < r.nrhs; ++j)
@@ -421,7 +426,7 @@ int check_request(const wl_solve_request *r) {
     for (int i = 0; i < 4; ++i) {
         if (r->dims[i] < 2 || (r->dims[i] & 1))
             return fail(WL_E_ARG, "lattice size %d in direction %d: must be even and >= 2", r->dims[i], i);
-        if (!r->gauge[i]) return fail(WL_E_ARG, "null gauge direction %d", i);
+        if (!r->config_path && !r->gauge[i]) return fail(WL_E_ARG, "null gauge direction %d", i);
     }
     if (r->nrhs < 1 || r->nrhs > 64) return fail(WL_E_ARG, "nrhs %d out of range 1..64", r->nrhs);
     if (!r->source || !r->solution) return fail(WL_E_ARG, "null source or solution buffer");
@@ -477,6 +482,10 @@ extern "C" int wl_server_submit(wl_server *S, const wl_solve_request *req, uint6
     if (int rc = check_request(req)) return rc;
     auto q = std::make_shared<Request>();
     q->r = *req;
+    if (req->config_path) {
+        q->config_path = req->config_path;
+        q->r.config_path = q->config_path.c_str();
+    }
     q->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(S->mu);

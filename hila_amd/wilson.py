@@ -38,6 +38,7 @@ EXPORTED_SYMBOLS = [
     "wl_cg_solve", "wl_cg_solve_mixed", "wl_set_option", "wl_get_option", "wl_time_op", "wl_gen_gauge", "wl_gen_spinor", "wl_gen_point",
     "wl_server_create", "wl_server_submit", "wl_server_wait", "wl_server_info_get", "wl_server_destroy",
     "wl_host_alloc", "wl_host_free", "wl_autotune", "wl_server_set_option",
+    "wl_gauge_download", "wl_config_header", "wl_config_read", "wl_config_write",
 ]
 
 
@@ -60,7 +61,8 @@ class SolveRequest(C.Structure):
     _fields_ = [("dims", C.c_int * 4), ("antiperiodic_t", C.c_int), ("nrhs", C.c_int),
                 ("gauge", C.c_void_p * 4), ("source", C.c_void_p), ("solution", C.c_void_p),
                 ("kappa", C.c_double), ("tolerance", C.c_double), ("maxiters", C.c_int),
-                ("inner_prec", C.c_int), ("delta", C.c_double), ("update_every", C.c_int)]
+                ("inner_prec", C.c_int), ("delta", C.c_double), ("update_every", C.c_int),
+                ("config_path", C.c_char_p)]
 
 
 class SolveResult(C.Structure):
@@ -218,6 +220,22 @@ class Lattice:
             raise ValueError(f"gauge must have shape (4, {self.V}, 18)")
         ptrs = (C.c_void_p * 4)(*[U[m].ctypes.data for m in range(4)])
         check(lib().wl_gauge_upload(self.handle, ptrs))
+
+    def download_gauge(self):
+        """The device gauge field as (4, V, 18) float64, HILA layout (wl_gauge_download)."""
+        U = np.empty((4, self.V, 18), dtype=np.float64)
+        ptrs = (C.c_void_p * 4)(*[U[m].ctypes.data for m in range(4)])
+        check(lib().wl_gauge_download(self.handle, ptrs))
+        return U
+
+    def config_read(self, path):
+        """GaugeField::config_read (gaugefield.h:219-272): the gauge field from a HILA
+        configuration file, replicated when the file's lattice divides this one."""
+        check(lib().wl_config_read(self.handle, os.fsencode(path)))
+
+    def config_write(self, path):
+        """GaugeField::config_write (gaugefield.h:194-217)."""
+        check(lib().wl_config_write(self.handle, os.fsencode(path)))
 
     def sync(self):
         check(lib().wl_sync(self.handle))
@@ -431,12 +449,16 @@ class Server:
 
     def submit(self, dims, gauge, source, solution, kappa, tolerance, maxiters=10000,
                precision="f32", delta=0.1, update_every=100, antiperiodic_t=True):
-        """gauge: (4, V, 18) float64; source, solution: (nrhs, V/2, 24) float64 (even sites).
-        precision 'f64' (FP64 CG), 'f32' or 'f16' (mixed, reliable updates).  Returns a ticket."""
+        """gauge: (4, V, 18) float64, or the path of a HILA configuration file; source,
+        solution: (nrhs, V/2, 24) float64 (even sites).  precision 'f64' (FP64 CG), 'f32' or
+        'f16' (mixed, reliable updates).  Returns a ticket."""
         dims = tuple(int(x) for x in dims)
         V = int(np.prod(dims))
-        if gauge.shape != (4, V, 18) or gauge.dtype != np.float64 or not gauge.flags.c_contiguous:
-            raise ValueError("gauge must be a C-contiguous float64 array (4, V, 18)")
+        config = None
+        if isinstance(gauge, (str, bytes, os.PathLike)):
+            config = os.fsencode(gauge)
+        elif gauge.shape != (4, V, 18) or gauge.dtype != np.float64 or not gauge.flags.c_contiguous:
+            raise ValueError("gauge must be a C-contiguous float64 array (4, V, 18) or a path")
         for a in (source, solution):
             if a.ndim != 3 or a.shape[1:] != (V // 2, 24) or a.dtype != np.float64 or not a.flags.c_contiguous:
                 raise ValueError("source/solution must be C-contiguous float64 arrays (nrhs, V/2, 24)")
@@ -446,8 +468,11 @@ class Server:
         r.dims = (C.c_int * 4)(*dims)
         r.antiperiodic_t = 1 if antiperiodic_t else 0
         r.nrhs = int(source.shape[0])
-        for m in range(4):
-            r.gauge[m] = gauge[m].ctypes.data
+        if config is None:
+            for m in range(4):
+                r.gauge[m] = gauge[m].ctypes.data
+        else:
+            r.config_path = config
         r.source = source.ctypes.data
         r.solution = solution.ctypes.data
         r.kappa = float(kappa)

@@ -77,6 +77,17 @@ class GaugeField {
         const double *p[4] = {U_[0].data(), U_[1].data(), U_[2].data(), U_[3].data()};
         check(wl_gauge_upload(lat_.ctx(), p), "wl_gauge_upload");
     }
+    // GaugeField::config_write / config_read (libraries/plumbing/gaugefield.h:194-272): HILA's
+    // configuration file, read straight into the device gauge field (replicated when the file's
+    // lattice divides this one); the host copy is refreshed from the device.
+    void config_write(const std::string &filename) const {
+        check(wl_config_write(lat_.ctx(), filename.c_str()), "wl_config_write");
+    }
+    void config_read(const std::string &filename) {
+        check(wl_config_read(lat_.ctx(), filename.c_str()), "wl_config_read");
+        double *p[4] = {U_[0].data(), U_[1].data(), U_[2].data(), U_[3].data()};
+        check(wl_gauge_download(lat_.ctx(), p), "wl_gauge_download");
+    }
     Lattice &lattice() { return lat_; }
 
   private:

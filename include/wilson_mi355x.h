@@ -101,6 +101,29 @@ int64_t wl_volume(const wl_ctx *ctx);
  * (V sites x 18 doubles, EVEN_SITES_FIRST order).  Replaces the non-owning gauge reference
  * that Dirac_Wilson_evenodd holds (wilson.h:237,261). */
 int wl_gauge_upload(wl_ctx *ctx, const double *const U[4]);
+/* The gauge field back in HILA's layout (U[mu]: V x 18 doubles, EVEN_SITES_FIRST), as it is
+ * held on the device: what a GaugeField read from a file holds (gaugefield.h:219-272). */
+int wl_gauge_download(wl_ctx *ctx, double *const U[4]);
+
+/* ---- gauge configuration files ------------------------------------------------------ */
+
+/* HILA's configuration file of a GaugeField<SU<3,double>> (libraries/plumbing/gaugefield.h:
+ * 194-272): 7 native-endian int64 {394824242, 4, 144, Lx, Ly, Lz, Lt}, then for mu = 0..3 the
+ * link U_mu(x) of every site in lexicographic order with x fastest (lattice.h:389-396,
+ * Field<T>::write field_io.h:125-160), each 18 doubles (3x3 complex, row-major).  Errors are
+ * WL_E_ARG with HILA's "CONFIG ERROR in file ..." text where config_read would terminate. */
+
+/* The lattice size in the header of a configuration file, after config_read's checks of the
+ * id, the dimensionality and the element size (gaugefield.h:219-262), and of the file's
+ * length. */
+int wl_config_header(const char *path, int dims[4]);
+/* GaugeField::config_read (gaugefield.h:219-272): load the file's links as the context's
+ * gauge field.  Every file dimension must divide the lattice's; a smaller file lattice is
+ * replicated periodically to fill it (Field<T>::read(stream, insize), field_io.h:245-312).
+ * A failure after the header checks leaves the context without a gauge field. */
+int wl_config_read(wl_ctx *ctx, const char *path);
+/* GaugeField::config_write (gaugefield.h:194-217): write the context's gauge field. */
+int wl_config_write(wl_ctx *ctx, const char *path);
 
 /* ---- device fields (one parity, nrhs right-hand sides) -------------------------------- */
 
@@ -264,6 +287,9 @@ typedef struct wl_solve_request {
     int inner_prec;              /* WL_F64: FP64 CG; WL_F32: mixed precision, reliable updates */
     double delta;                /* mixed: update when ||r|| drops by delta (0 < delta < 1)   */
     int update_every;            /* mixed: ... or after this many iterations                  */
+    const char *config_path;     /* if not null: the gauge field is read from this HILA
+                                    configuration file (wl_config_read) and gauge[] is unused;
+                                    the path is copied at submission                          */
 } wl_solve_request;
 
 typedef struct wl_solve_result {

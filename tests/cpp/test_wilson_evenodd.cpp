@@ -7,6 +7,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <unistd.h>
 #include <vector>
 
 #include "hila_wilson.hpp"
@@ -103,6 +104,29 @@ static void run(Lattice &lattice, int link_kind, const char *name) {
         CHECK(mixed.stats.converged == 1 && mixed.stats.true_resid <= 1e-10,
               prec == WL_F32 ? "mixed CG (FP32 inner) true residual" : "mixed CG (FP16 inner) true residual");
         CHECK(diffre < 1e-14 * b.squarenorm(EVEN), "mixed (DdgD)^-1 DdgD");
+    }
+
+    // GaugeField::config_write then config_read (gaugefield.h:194-272, as checkpoint.h:30,89)
+    // gives back the same links, and the operator on them the same output, to the last bit
+    char path[] = "/tmp/wl_config_XXXXXX";
+    const int fd = mkstemp(path);
+    CHECK(fd >= 0, "temporary configuration file");
+    if (fd >= 0) {
+        close(fd);
+        U.config_write(path);
+        GaugeField U2(lattice);
+        U2.config_read(path);
+        std::remove(path);
+        bool same = true;
+        for (int mu = 0; mu < 4; ++mu) same = same && U2.direction(mu) == U.direction(mu);
+        CHECK(same, "config_write / config_read round trip");
+        U.upload();
+        WilsonField Db1(lattice), Db2(lattice);
+        D.apply(b, Db1);
+        dirac D2(0.12, U2);
+        U2.upload();
+        D2.apply(b, Db2);
+        CHECK(Db1.host() == Db2.host(), "operator on the configuration read back");
     }
 
     // apply leaves the odd half of the output at zero (wilson.h:278)
