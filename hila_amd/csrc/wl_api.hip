@@ -151,6 +151,7 @@ int dslash_one(wl_ctx *c, int prec, void *out, int out_par, const void *in, cons
     L.xin = xin;
     L.gauge = gauge_for(c, prec);
     L.a = a;
+    L.b = 1.0 + c->op_shift;
     L.out_par = out_par;
     L.sign = sign;
     L.epi = epi;
@@ -256,6 +257,7 @@ int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, c
         L.xin = xin;
         L.gauge = gauge_for(c, prec);
         L.a = a;
+        L.b = 1.0 + c->op_shift;
         L.out_par = out_par;
         L.sign = sign;
         L.epi = epi;
@@ -565,6 +567,24 @@ extern "C" int wl_field_copy(wl_field *dst, const wl_field *src) {
     return WL_OK;
 }
 
+extern "C" int wl_field_copy_rhs(wl_field *dst, int jd, const wl_field *src, int js) {
+    if (!dst || !src) return fail(WL_E_ARG, "null argument");
+    if (!dst->ctx || !src->ctx) return fail(WL_E_STATE, "the field's context was destroyed");
+    if (dst->ctx != src->ctx) return fail(WL_E_ARG, "fields belong to different contexts");
+    if (dst->parity != src->parity || dst->prec != src->prec)
+        return fail(WL_E_ARG, "fields differ in parity or precision");
+    if (jd < 0 || jd >= dst->nrhs || js < 0 || js >= src->nrhs)
+        return fail(WL_E_ARG, "rhs %d of %d <- rhs %d of %d out of range", jd, dst->nrhs, js, src->nrhs);
+    if (dst == src && jd == js) return WL_OK;
+    wl_ctx *c = dst->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    const size_t bytes = dst->rhs_elems * prec_bytes(dst->prec);  // one rhs is one contiguous block
+    HIPCHK(hipMemcpyAsync((char *)dst->data + (size_t)jd * bytes, (const char *)src->data + (size_t)js * bytes, bytes,
+                          hipMemcpyDeviceToDevice, c->stream));
+    return WL_OK;
+}
+
 // ================================================================== operator ============
 
 extern "C" int wl_dslash_eo(wl_ctx *c, wl_field *out, const wl_field *in, double kappa, int sign) {
@@ -586,7 +606,11 @@ extern "C" int wl_dslash_eo(wl_ctx *c, wl_field *out, const wl_field *in, double
                          in->nrhs, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
-extern "C" int wl_meo_apply(wl_ctx *c, wl_field *out, const wl_field *in, double kappa, int sign) {
+namespace {
+
+// M_sign + mh on every rhs of in (Dirac_Wilson_evenodd::apply/dagger, wilson.h:272-289, with
+// Hasenbusch_operator's shift, Hasenbusch.h:40-48)
+int meo_apply(wl_ctx *c, wl_field *out, const wl_field *in, double kappa, double mh, int sign) {
     if (!c || !out || !in) return fail(WL_E_ARG, "null argument");
     if (check_sign(sign)) return WL_E_ARG;
     if (out->ctx != c || in->ctx != c) return fail(WL_E_ARG, "field belongs to another context");
@@ -594,8 +618,10 @@ extern "C" int wl_meo_apply(wl_ctx *c, wl_field *out, const wl_field *in, double
     if (out == in) return fail(WL_E_ARG, "out and in must be different fields");
     if (out->prec != in->prec || out->nrhs != in->nrhs)
         return fail(WL_E_ARG, "out and in differ in precision or nrhs");
+    if (!std::isfinite(mh)) return fail(WL_E_ARG, "the shift must be finite");
     std::lock_guard<std::mutex> lk(c->mu);
     DeviceGuard dg(c->device);
+    OpShift os(c, mh);
     int rc = ensure_gauge(c, in->prec);
     if (rc) return rc;
     if (in->nrhs == 1 && in->prec != WL_F16) {
@@ -609,6 +635,17 @@ extern "C" int wl_meo_apply(wl_ctx *c, wl_field *out, const wl_field *in, double
     if ((rc = work_field(c, W_T, in->prec, WL_ODD, in->nrhs, &t))) return rc;
     return meo_mr(c, in->prec, out->data, in->data, t->data, kappa, sign, EPI_AXPY, in->nrhs,
                   nullptr, nullptr, nullptr);
+}
+
+}  // namespace
+
+extern "C" int wl_meo_apply(wl_ctx *c, wl_field *out, const wl_field *in, double kappa, int sign) {
+    return meo_apply(c, out, in, kappa, 0.0, sign);
+}
+
+extern "C" int wl_hasenbusch_apply(wl_ctx *c, wl_field *out, const wl_field *in, double kappa, double mh,
+                                   int sign) {
+    return meo_apply(c, out, in, kappa, mh, sign);
 }
 
 // ================================================================ diagnostics ===========

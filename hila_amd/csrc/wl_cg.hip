@@ -79,6 +79,7 @@ int cg_f64(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int
             L.xin = Dp;
             L.gauge = c->gauge64;
             L.a = kappa;
+            L.b = 1.0 + c->op_shift;
             L.out_par = 0;
             L.sign = -1;
             L.epi = EPI_CGUPD;
@@ -201,6 +202,7 @@ int cg_mixed(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, i
             L.xin = Dp;
             L.gauge = gauge_for(c, prec);
             L.a = kappa;
+            L.b = 1.0 + c->op_shift;
             L.out_par = 0;
             L.sign = -1;
             L.epi = EPI_CGUPD;
@@ -296,6 +298,7 @@ int BatchCG::begin(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec)
     b_ = b;
     N_ = N;
     s_ = spec;
+    c->op_shift = s_.mh;  // every launch this solve enqueues applies M + mh
     chunks_ = 0;
     stop_ = final_enqueued_ = done_ = false;
     inflight_.clear();
@@ -483,6 +486,7 @@ int BatchCG::enqueue_final() {
 int BatchCG::pump(bool *finished) {
     *finished = done_;
     if (done_) return WL_OK;
+    c_->op_shift = s_.mh;
     while (!inflight_.empty()) {
         const Check ck = inflight_.front();
         const hipError_t q = hipEventQuery(ck.ev);
@@ -554,56 +558,74 @@ int cg_batched(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec, wl_
 
 using namespace wl;
 
-extern "C" int wl_cg_solve(wl_ctx *c, wl_field *x, const wl_field *b, double kappa, double accuracy,
-                           int maxiters, wl_cg_stats *stats) {
-    if (!c || !x || !b) return fail(WL_E_ARG, "null argument");
-    if (x->ctx != c || b->ctx != c) return fail(WL_E_ARG, "field belongs to another context");
-    if (x->parity != WL_EVEN || b->parity != WL_EVEN) return fail(WL_E_ARG, "CG acts on even fields");
-    if (x == b) return fail(WL_E_ARG, "x and b must be different fields");
-    if (x->prec != WL_F64 || b->prec != WL_F64) return fail(WL_E_ARG, "wl_cg_solve takes FP64 fields");
-    if (x->nrhs != b->nrhs) return fail(WL_E_ARG, "x and b differ in nrhs");
-    if (!(accuracy > 0)) return fail(WL_E_ARG, "accuracy must be > 0");
-    std::lock_guard<std::mutex> lk(c->mu);
-    DeviceGuard dg(c->device);
-    int rc = ensure_gauge(c, WL_F64);
-    if (rc) return rc;
-    if (b->nrhs == 1) return cg_f64(c, x->data, b->data, kappa, accuracy, maxiters, stats);
-    CgSpec s;
-    s.kappa = kappa;
-    s.accuracy = accuracy;
-    s.maxiters = maxiters;
-    s.prec = WL_F64;
-    s.mixed = false;
-    return cg_batched(c, x->data, b->data, b->nrhs, s, stats);
-}
+namespace {
 
-extern "C" int wl_cg_solve_mixed(wl_ctx *c, wl_field *x, const wl_field *b, double kappa,
-                                 double accuracy, int maxiters, int inner_prec, double delta,
-                                 int update_every, wl_cg_stats *stats) {
+// CG<Op>::apply (conjugate_gradient.h:46-104) for Op = Dirac_Wilson_evenodd (mh = 0) or
+// Hasenbusch_operator<Dirac_Wilson_evenodd> (Hasenbusch.h:15-53): plain FP64, or with reliable
+// updates (mixed) around an inner precision.
+int cg_solve(wl_ctx *c, wl_field *x, const wl_field *b, const wl_cg_params &p, bool mixed, wl_cg_stats *stats) {
     if (!c || !x || !b) return fail(WL_E_ARG, "null argument");
     if (x->ctx != c || b->ctx != c) return fail(WL_E_ARG, "field belongs to another context");
     if (x->parity != WL_EVEN || b->parity != WL_EVEN) return fail(WL_E_ARG, "CG acts on even fields");
     if (x == b) return fail(WL_E_ARG, "x and b must be different fields");
     if (x->prec != WL_F64 || b->prec != WL_F64) return fail(WL_E_ARG, "x and b must be FP64 fields");
     if (x->nrhs != b->nrhs) return fail(WL_E_ARG, "x and b differ in nrhs");
-    if (inner_prec != WL_F32 && inner_prec != WL_F64 && inner_prec != WL_F16)
-        return fail(WL_E_NOTIMPL, "inner precision %d", inner_prec);
-    if (!(accuracy > 0) || !(delta > 0 && delta < 1) || update_every < 1)
-        return fail(WL_E_ARG, "need accuracy > 0, 0 < delta < 1, update_every >= 1");
+    if (!(p.accuracy > 0)) return fail(WL_E_ARG, "accuracy must be > 0");
+    if (!std::isfinite(p.mh) || !std::isfinite(p.kappa)) return fail(WL_E_ARG, "kappa and the shift must be finite");
+    if (mixed) {
+        if (p.inner_prec != WL_F32 && p.inner_prec != WL_F64 && p.inner_prec != WL_F16)
+            return fail(WL_E_NOTIMPL, "inner precision %d", p.inner_prec);
+        if (!(p.delta > 0 && p.delta < 1) || p.update_every < 1)
+            return fail(WL_E_ARG, "need 0 < delta < 1, update_every >= 1");
+    }
     std::lock_guard<std::mutex> lk(c->mu);
     DeviceGuard dg(c->device);
+    OpShift os(c, p.mh);
     int rc = ensure_gauge(c, WL_F64);
     if (rc) return rc;
-    if (b->nrhs == 1 && inner_prec != WL_F16)  // FP16 inner: the batched solver, any count
-        return cg_mixed(c, x->data, b->data, kappa, accuracy, maxiters, inner_prec, delta,
-                        update_every, stats);
+    if (!mixed && b->nrhs == 1) return cg_f64(c, x->data, b->data, p.kappa, p.accuracy, p.maxiters, stats);
+    if (mixed && b->nrhs == 1 && p.inner_prec != WL_F16)  // FP16 inner: the batched solver, any count
+        return cg_mixed(c, x->data, b->data, p.kappa, p.accuracy, p.maxiters, p.inner_prec, p.delta,
+                        p.update_every, stats);
     CgSpec s;
-    s.kappa = kappa;
-    s.accuracy = accuracy;
-    s.maxiters = maxiters;
-    s.prec = inner_prec;
-    s.mixed = true;
-    s.delta = delta;
-    s.update_every = update_every;
+    s.kappa = p.kappa;
+    s.mh = p.mh;
+    s.accuracy = p.accuracy;
+    s.maxiters = p.maxiters;
+    s.prec = mixed ? p.inner_prec : WL_F64;
+    s.mixed = mixed;
+    s.delta = p.delta;
+    s.update_every = p.update_every;
     return cg_batched(c, x->data, b->data, b->nrhs, s, stats);
+}
+
+}  // namespace
+
+extern "C" int wl_cg_solve(wl_ctx *c, wl_field *x, const wl_field *b, double kappa, double accuracy,
+                           int maxiters, wl_cg_stats *stats) {
+    wl_cg_params p{};
+    p.kappa = kappa;
+    p.accuracy = accuracy;
+    p.maxiters = maxiters;
+    p.inner_prec = WL_F64;
+    return cg_solve(c, x, b, p, false, stats);
+}
+
+extern "C" int wl_cg_solve_mixed(wl_ctx *c, wl_field *x, const wl_field *b, double kappa,
+                                 double accuracy, int maxiters, int inner_prec, double delta,
+                                 int update_every, wl_cg_stats *stats) {
+    wl_cg_params p{};
+    p.kappa = kappa;
+    p.accuracy = accuracy;
+    p.maxiters = maxiters;
+    p.inner_prec = inner_prec;
+    p.delta = delta;
+    p.update_every = update_every;
+    return cg_solve(c, x, b, p, true, stats);
+}
+
+extern "C" int wl_cg_solve_ex(wl_ctx *c, wl_field *x, const wl_field *b, const wl_cg_params *p,
+                              wl_cg_stats *stats) {
+    if (!p) return fail(WL_E_ARG, "null argument");
+    return cg_solve(c, x, b, *p, p->inner_prec != WL_F64, stats);
 }

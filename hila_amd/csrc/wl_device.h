@@ -33,6 +33,22 @@ template <typename R>
 __device__ __forceinline__ cx<R> operator*(R s, cx<R> a) {
     return {s * a.x, s * a.y};
 }
+// v *= b for a spinor.  Each product is rounded on its own (an fma with a zero addend, which
+// the compiler does not fuse with the addition that follows), so with b == 1 the values, and
+// every later rounding, are exactly those of the unscaled spinor.  (A branch on b == 1 would
+// keep both versions live and spill the FP64 kernels.)
+template <typename R>
+__device__ __forceinline__ void scale_unless_one(cx<R> (&v)[4][3], R b) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if constexpr (sizeof(R) == 8)
+                v[s][c] = {__builtin_fma(b, v[s][c].x, 0.0), __builtin_fma(b, v[s][c].y, 0.0)};
+            else
+                v[s][c] = {__builtin_fmaf(b, v[s][c].x, 0.0f), __builtin_fmaf(b, v[s][c].y, 0.0f)};
+        }
+}
 // a * b
 template <typename R>
 __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {

@@ -32,6 +32,7 @@ struct DslashArgs {
     const cx<R> *__restrict__ xin;
     const cx<R> *__restrict__ gauge;
     R a;
+    R b;    // coefficient of xin (1 + the Hasenbusch shift in the second half of M)
     Geom g;
     int out_par;
     const int *done;
@@ -63,6 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
         } else if constexpr (EPI == EPI_AXPY || EPI == EPI_AXPY_NORM) {
             cx<R> v[4][3];
             ld_site<R, true>(A.xin, S, k, v);
+            scale_unless_one(v, A.b);
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -76,6 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
             const R alpha_r = (R)alpha;
             cx<R> ddp[4][3], pv[4][3], rv[4][3];
             ld_site<R, true>(A.xin, S, k, ddp);
+            scale_unless_one(ddp, A.b);
             ld_site<R, true>(A.p, S, k, pv);
             ld_site<R, true>(A.r, S, k, rv);
             cx<double> xv[4][3];
@@ -115,6 +118,7 @@ static hipError_t launch_t(const Geom &g, const DslashLaunch &L, hipStream_t s) 
     A.xin = (const cx<R> *)L.xin;
     A.gauge = (const cx<R> *)L.gauge;
     A.a = (R)L.a;
+    A.b = (R)L.b;
     A.g = g;
     A.out_par = L.out_par;
     A.done = L.done;

@@ -34,6 +34,17 @@ struct DeviceGuard {
     }
 };
 
+// The operator shift (Hasenbusch_operator's h_parameter, Hasenbusch.h:40-48) that the
+// launches of one entry-point call apply: every second half of M computes
+// (1 + mh) in + kappa H_eo tmp, i.e. M + mh.  Reset to 0 when the call returns.
+struct OpShift {
+    wl_ctx *c;
+    OpShift(wl_ctx *ctx, double mh) : c(ctx) { c->op_shift = mh; }
+    ~OpShift() { c->op_shift = 0.0; }
+    OpShift(const OpShift &) = delete;
+    OpShift &operator=(const OpShift &) = delete;
+};
+
 // Device allocations of a context, counted in wl_ctx::dev_bytes.
 int dev_alloc(wl_ctx *c, void **p, size_t bytes, const char *what);
 void dev_free(wl_ctx *c, void *p, size_t bytes);

@@ -39,6 +39,7 @@ struct DslashLaunch {
     const void *xin;
     const void *gauge;
     double a;
+    double b = 1.0;  // coefficient of xin in the epilogues that read it: out = b xin + a H in
     int out_par;   // 0 even, 1 odd
     int sign;      // +1 apply, -1 dagger
     int epi;
@@ -91,6 +92,7 @@ struct MrLaunch {
     const void *xin;
     const void *gauge;
     double a;
+    double b = 1.0;           // coefficient of xin (see DslashLaunch)
     int out_par, sign, epi;
     int j0, NB, NR;
     bool link_nt;             // nontemporal link loads
@@ -189,6 +191,8 @@ struct wl_ctx {
     int link_load = 0;                 // multi-rhs links: 0 automatic, 1 cached, 2 nontemporal, 3 LDS
     bool use_graphs = false;           // batched CG: replay each chunk from a captured HIP graph
     int pipeline = 0;                  // multi-rhs hops: 0 pipelined where possible, 1 never
+    double op_shift = 0.0;             // Hasenbusch shift mh of the operator being applied,
+                                       // M + mh (0 outside a shifted call; see OpShift)
     // fields created through wl_field_create and not yet destroyed; wl_destroy orphans them
     std::unordered_set<wl_field *> user_fields;
     size_t dev_bytes = 0;              // device memory held by the context and its fields

@@ -38,6 +38,7 @@ struct RbArgs {
     const E *__restrict__ xin;
     const E *__restrict__ gauge;
     R a;
+    R b;                       // coefficient of xin (1 + the Hasenbusch shift)
     Geom g;
     int out_par;
     int j0, jend, C;           // right-hand sides [j0, jend) in chunks of NR; C chunks
@@ -228,6 +229,7 @@ __device__ __forceinline__ void rb_epilogue(const RbArgs<T> &A, cx<typename St<T
         } else if constexpr (EPI == EPI_AXPY || EPI == EPI_AXPY_NORM) {
             cx<R> v[4][3];
             ld_site<T, true>(A.xin + eb, S, k, v);
+            scale_unless_one(v, A.b);
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -244,6 +246,7 @@ __device__ __forceinline__ void rb_epilogue(const RbArgs<T> &A, cx<typename St<T
             const R alpha_r = (R)alpha;
             cx<R> ddp[4][3], pv[4][3], rv[4][3];
             ld_site<T, true>(A.xin + eb, S, k, ddp);
+            scale_unless_one(ddp, A.b);
             ld_site<T, true>(A.p + eb, S, k, pv);
             ld_site<T, true>(A.r + eb, S, k, rv);
             const size_t xb = (size_t)(jf + r) * 12 * S + k;
@@ -436,6 +439,7 @@ inline hipError_t rb_launch(const Geom &g, const MrLaunch &L, hipStream_t s) {
     A.xin = (const E *)L.xin;
     A.gauge = (const E *)L.gauge;
     A.a = (R)L.a;
+    A.b = (R)L.b;
     A.g = g;
     A.out_par = L.out_par;
     A.j0 = L.j0;
@@ -479,6 +483,7 @@ inline hipError_t rb_launch_lds(const Geom &g, const MrLaunch &L, hipStream_t s)
     A.xin = (const E *)L.xin;
     A.gauge = (const E *)L.gauge;
     A.a = (R)L.a;
+    A.b = (R)L.b;
     A.g = g;
     A.out_par = L.out_par;
     A.j0 = L.j0;

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -337,6 +338,7 @@ void scheduler_main(wl_server *S) {
             c->g.blk = blk;
             CgSpec spec;
             spec.kappa = r.kappa;
+            spec.mh = r.mh;
             spec.accuracy = r.tolerance;
             spec.maxiters = r.maxiters;
             spec.prec = r.inner_prec;
@@ -431,6 +433,7 @@ int check_request(const wl_solve_request *r) {
     if (r->nrhs < 1 || r->nrhs > 64) return fail(WL_E_ARG, "nrhs %d out of range 1..64", r->nrhs);
     if (!r->source || !r->solution) return fail(WL_E_ARG, "null source or solution buffer");
     if (!(r->tolerance > 0)) return fail(WL_E_ARG, "tolerance must be > 0");
+    if (!std::isfinite(r->mh) || !std::isfinite(r->kappa)) return fail(WL_E_ARG, "kappa and mh must be finite");
     if (r->inner_prec != WL_F64 && r->inner_prec != WL_F32 && r->inner_prec != WL_F16)
         return fail(WL_E_NOTIMPL, "inner precision %d", r->inner_prec);
     if (r->inner_prec != WL_F64 && (!(r->delta > 0 && r->delta < 1) || r->update_every < 1))

@@ -29,6 +29,7 @@ constexpr int kChunk = 8;  // iterations per chunk (one host check each)
 
 struct CgSpec {
     double kappa = 0.125;
+    double mh = 0.0;       // Hasenbusch shift: CG on (M + mh)^dag (M + mh)
     double accuracy = 1e-12;
     int maxiters = 10000;
     int prec = WL_F64;     // precision of the inner operator and Krylov vectors

@@ -39,6 +39,7 @@ EXPORTED_SYMBOLS = [
     "wl_server_create", "wl_server_submit", "wl_server_wait", "wl_server_info_get", "wl_server_destroy",
     "wl_host_alloc", "wl_host_free", "wl_autotune", "wl_server_set_option",
     "wl_gauge_download", "wl_config_header", "wl_config_read", "wl_config_write",
+    "wl_hasenbusch_apply", "wl_cg_solve_ex", "wl_mre_guess", "wl_field_copy_rhs",
 ]
 
 
@@ -62,7 +63,13 @@ class SolveRequest(C.Structure):
                 ("gauge", C.c_void_p * 4), ("source", C.c_void_p), ("solution", C.c_void_p),
                 ("kappa", C.c_double), ("tolerance", C.c_double), ("maxiters", C.c_int),
                 ("inner_prec", C.c_int), ("delta", C.c_double), ("update_every", C.c_int),
-                ("config_path", C.c_char_p)]
+                ("config_path", C.c_char_p), ("mh", C.c_double)]
+
+
+class CGParams(C.Structure):
+    _fields_ = [("kappa", C.c_double), ("mh", C.c_double), ("accuracy", C.c_double),
+                ("maxiters", C.c_int), ("inner_prec", C.c_int), ("delta", C.c_double),
+                ("update_every", C.c_int)]
 
 
 class SolveResult(C.Structure):
@@ -139,6 +146,14 @@ def lib():
         L.wl_host_free.argtypes = [vp]
         L.wl_autotune.argtypes = [vp, i, i, C.POINTER(TuneResult)]
         L.wl_server_set_option.argtypes = [vp, i, i]
+        L.wl_gauge_download.argtypes = [vp, C.POINTER(vp)]
+        L.wl_config_header.argtypes = [C.c_char_p, ip4]
+        L.wl_config_read.argtypes = [vp, C.c_char_p]
+        L.wl_config_write.argtypes = [vp, C.c_char_p]
+        L.wl_hasenbusch_apply.argtypes = [vp, vp, vp, d, d, i]
+        L.wl_cg_solve_ex.argtypes = [vp, vp, vp, C.POINTER(CGParams), C.POINTER(CGStats)]
+        L.wl_mre_guess.argtypes = [vp, vp, vp, vp, i, d, d]
+        L.wl_field_copy_rhs.argtypes = [vp, i, vp, i]
         for name in EXPORTED_SYMBOLS:
             if name not in ("wl_version", "wl_last_error", "wl_volume"):
                 getattr(L, name).restype = C.c_int
@@ -327,6 +342,11 @@ class Field:
         check(lib().wl_field_copy(self._h, src.handle))
         return self
 
+    def copy_rhs_from(self, rhs, src: "Field", src_rhs=0):
+        """Right-hand side `rhs` of self = right-hand side `src_rhs` of src."""
+        check(lib().wl_field_copy_rhs(self._h, int(rhs), src.handle, int(src_rhs)))
+        return self
+
     def norm2(self, rhs=0):
         d = C.c_double(0)
         check(lib().wl_field_norm2(self._h, rhs, C.byref(d)))
@@ -355,6 +375,8 @@ class Dirac_Wilson_evenodd:
         self.kappa = float(kappa)
         self.lat = gauge
 
+    mh = 0.0
+
     def apply(self, fin: Field, fout: Field):
         check(lib().wl_meo_apply(self.lat.handle, fout.handle, fin.handle, self.kappa, 1))
 
@@ -374,8 +396,41 @@ class Dirac_Wilson_evenodd:
         return ms.value
 
 
+class Hasenbusch_operator:
+    """HILA's Hasenbusch_operator<Dirac_Wilson_evenodd> (Hasenbusch.h:15-53): D + mh, with
+    apply / dagger = (M + mh) / (M^dag + mh) on even sites."""
+
+    par = EVEN
+
+    def __init__(self, D: Dirac_Wilson_evenodd, hasenbusch_parameter):
+        self.D = D
+        self.kappa = D.kappa
+        self.lat = D.lat
+        self.mh = float(hasenbusch_parameter)
+        self.h_parameter = self.mh
+
+    def apply(self, fin: Field, fout: Field):
+        check(lib().wl_hasenbusch_apply(self.lat.handle, fout.handle, fin.handle, self.kappa,
+                                        self.mh, 1))
+
+    def dagger(self, fin: Field, fout: Field):
+        check(lib().wl_hasenbusch_apply(self.lat.handle, fout.handle, fin.handle, self.kappa,
+                                        self.mh, -1))
+
+
+def MRE_guess(psi: Field, chi: Field, D, basis: Field, nbasis=None):
+    """HILA's MRE_guess(psi, chi, D, old_chi_inv) (libraries/hmc/MRE_guess.h:8-87): psi =
+    the minimal-residual combination, for A = D^dag D, of the first nbasis right-hand sides of
+    `basis` (previous solutions).  D is a Dirac_Wilson_evenodd or Hasenbusch_operator."""
+    n = basis.nrhs if nbasis is None else int(nbasis)
+    check(lib().wl_mre_guess(D.lat.handle, psi.handle, chi.handle, basis.handle, n, D.kappa,
+                             getattr(D, "mh", 0.0)))
+    return psi
+
+
 class CG:
-    """HILA's CG<Op> (conjugate_gradient.h:19-105): solves (M^dag M) out = in.
+    """HILA's CG<Op> (conjugate_gradient.h:19-105): solves (M^dag M) out = in, for Op a
+    Dirac_Wilson_evenodd or a Hasenbusch_operator of one (then M + mh).
 
     precision="f64" runs HILA's algorithm in FP64.  precision="f32" runs the mixed-precision
     solver with reliable updates (inner operator and Krylov vectors in FP32, solution and
@@ -383,7 +438,7 @@ class CG:
     precision="f16" the same with the inner fields stored in FP16 with a per-site scale (FP32
     arithmetic); delta and update_every set when reliable updates happen."""
 
-    def __init__(self, op: Dirac_Wilson_evenodd, accuracy=1e-12, maxiters=10000,
+    def __init__(self, op, accuracy=1e-12, maxiters=10000,
                  precision="f64", delta=1e-3, update_every=100):
         self.M = op
         self.accuracy = float(accuracy)
@@ -396,7 +451,17 @@ class CG:
         """Runs CG; fout holds the initial guess on entry.  Returns per-rhs stats dicts.
         Like HILA, reaching maxiters is not an error unless raise_on_noconv is set."""
         stats = (CGStats * fin.nrhs)()
-        if self.precision == "f64":
+        mh = getattr(self.M, "mh", 0.0)
+        if mh != 0.0:
+            if self.precision not in ("f64", "f32", "f16"):
+                raise ValueError(f"precision {self.precision!r} has no shifted-operator solver")
+            p = CGParams(kappa=self.M.kappa, mh=mh, accuracy=self.accuracy,
+                         maxiters=self.maxiters,
+                         inner_prec={"f64": F64, "f32": F32, "f16": F16}[self.precision],
+                         delta=self.delta, update_every=self.update_every)
+            rc = lib().wl_cg_solve_ex(self.M.lat.handle, fout.handle, fin.handle, C.byref(p),
+                                      stats)
+        elif self.precision == "f64":
             rc = lib().wl_cg_solve(self.M.lat.handle, fout.handle, fin.handle, self.M.kappa,
                                    self.accuracy, self.maxiters, stats)
         else:
@@ -448,7 +513,7 @@ class Server:
         self._keep = {}
 
     def submit(self, dims, gauge, source, solution, kappa, tolerance, maxiters=10000,
-               precision="f32", delta=0.1, update_every=100, antiperiodic_t=True):
+               precision="f32", delta=0.1, update_every=100, antiperiodic_t=True, mh=0.0):
         """gauge: (4, V, 18) float64, or the path of a HILA configuration file; source,
         solution: (nrhs, V/2, 24) float64 (even sites).  precision 'f64' (FP64 CG), 'f32' or
         'f16' (mixed, reliable updates).  Returns a ticket."""
@@ -481,6 +546,7 @@ class Server:
         r.inner_prec = {"f64": F64, "f32": F32, "f16": F16}[precision]
         r.delta = float(delta)
         r.update_every = int(update_every)
+        r.mh = float(mh)
         t = C.c_uint64(0)
         check(lib().wl_server_submit(self._h, C.byref(r), C.byref(t)))
         self._keep[t.value] = (gauge, source, solution)

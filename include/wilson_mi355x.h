@@ -145,6 +145,10 @@ int wl_field_zero(wl_field *f);
 /* dst = src (same parity, precision and nrhs): HILA's Field assignment dst[par] = src[X]
  * (libraries/plumbing/field.h:898-909), on the device. */
 int wl_field_copy(wl_field *dst, const wl_field *src);
+/* Right-hand side jd of dst = right-hand side js of src (same parity and precision): e.g.
+ * keeping a solution in a slot of a basis of previous solutions, as
+ * fermion_action::save_new_solution does (libraries/hmc/fermion_field.h:134-142). */
+int wl_field_copy_rhs(wl_field *dst, int jd, const wl_field *src, int js);
 /* sum over sites of ||f||^2 for one rhs, computed on the device in FP64 (the reduction of
  * conjugate_gradient.h:61,72,79,87). */
 int wl_field_norm2(const wl_field *f, int rhs, double *out);
@@ -159,6 +163,13 @@ int wl_dslash_eo(wl_ctx *ctx, wl_field *out, const wl_field *in, double kappa, i
 /* out = M_s in on even sites: sign=+1 is Dirac_Wilson_evenodd::apply (wilson.h:272-279),
  * sign=-1 is ::dagger (wilson.h:282-289).  in and out are even fields. */
 int wl_meo_apply(wl_ctx *ctx, wl_field *out, const wl_field *in, double kappa, int sign);
+
+/* out = (M_s + mh) in: Hasenbusch_operator<Dirac_Wilson_evenodd>::apply (s = +1) / ::dagger
+ * (s = -1), libraries/dirac/Hasenbusch.h:40-48, the operator D + mh of the Hasenbusch split of
+ * the fermion determinant.  The shift is applied inside the second hopping pass of M, so this
+ * costs the same as wl_meo_apply. */
+int wl_hasenbusch_apply(wl_ctx *ctx, wl_field *out, const wl_field *in, double kappa, double mh,
+                        int sign);
 
 /* ---- inverter ------------------------------------------------------------------------- */
 
@@ -183,6 +194,41 @@ int wl_cg_solve(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa, doubl
 int wl_cg_solve_mixed(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa, double accuracy,
                       int maxiters, int inner_prec, double delta, int update_every,
                       wl_cg_stats *stats);
+
+/* All solver settings at once. */
+typedef struct wl_cg_params {
+    double kappa;
+    double mh;           /* Hasenbusch shift: CG<Hasenbusch_operator<Dirac_Wilson_evenodd>>
+                            solves (M + mh)^dag (M + mh) x = b (Hasenbusch.h:15-53); 0: plain M */
+    double accuracy;
+    int maxiters;
+    int inner_prec;      /* WL_F64: FP64 CG as wl_cg_solve; WL_F32 / WL_F16: reliable updates
+                            as wl_cg_solve_mixed                                               */
+    double delta;        /* mixed only */
+    int update_every;    /* mixed only */
+} wl_cg_params;
+
+/* CG<Op>::apply(b, x) for Op = Dirac_Wilson_evenodd or Hasenbusch_operator of it, with the
+ * settings of *params; otherwise as wl_cg_solve / wl_cg_solve_mixed. */
+int wl_cg_solve_ex(wl_ctx *ctx, wl_field *x, const wl_field *b, const wl_cg_params *params,
+                   wl_cg_stats *stats);
+
+/* MRE_guess (libraries/hmc/MRE_guess.h:8-87): initial guess psi for CG on
+ * A = (M + mh)^dag (M + mh), A psi = chi, from previous solutions.  basis is an even FP64
+ * field whose right-hand sides 0..nbasis-1 hold them (HILA's old_chi_inv); chi and psi are
+ * even FP64 fields of one right-hand side.  psi = sum_i c_i basis_i with real c_i chosen as
+ * HILA's algorithm intends: Gram-Schmidt on the basis under Re <a, b> (Vector::rdot), then
+ * the Galerkin condition Re <basis_i, chi - A psi> = 0 for every kept basis vector, solved by
+ * Gauss-Jordan elimination.  Zero and linearly dependent basis vectors (and directions on
+ * which the projected A vanishes) are dropped, so the all-zero basis HILA starts with
+ * (fermion_field.h:38-44) gives psi = 0.  Three defects of the source text are not copied:
+ * its projection coefficient divides by the norm of the vector being orthogonalised instead
+ * of the one projected out (:24-28), its final loop overwrites psi with basis_i (1 + c_i)
+ * instead of accumulating c_i basis_i (:83-85), and save_new_solution's forward shift
+ * (fermion_field.h:135-141) leaves every older slot holding the same solution.  1 <= nbasis <= 8, or 0
+ * (psi = 0).  Cost: A on the nbasis vectors (one multi-rhs pass) plus two streaming passes. */
+int wl_mre_guess(wl_ctx *ctx, wl_field *psi, const wl_field *chi, const wl_field *basis, int nbasis,
+                 double kappa, double mh);
 
 /* ---- kernel options ------------------------------------------------------------------- */
 
@@ -290,6 +336,7 @@ typedef struct wl_solve_request {
     const char *config_path;     /* if not null: the gauge field is read from this HILA
                                     configuration file (wl_config_read) and gauge[] is unused;
                                     the path is copied at submission                          */
+    double mh;                   /* Hasenbusch shift: solve with M + mh (0: M)                */
 } wl_solve_request;
 
 typedef struct wl_solve_result {

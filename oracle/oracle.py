@@ -47,6 +47,10 @@ def lib(variant=0):
         for fn in ("oracle_hop", "oracle_hop_set", "oracle_hop_set_fused"):
             getattr(L, fn).argtypes = [vp, dp, C.c_double, dp, dp, C.c_int, C.c_int]
         L.oracle_meo.argtypes = [vp, dp, C.c_double, dp, dp, C.c_int]
+        L.oracle_hasenbusch.argtypes = [vp, dp, C.c_double, C.c_double, dp, dp, C.c_int]
+        L.oracle_cg_h.restype = C.c_int
+        L.oracle_cg_h.argtypes = [vp, dp, C.c_double, C.c_double, dp, dp, C.c_double, C.c_int,
+                                  C.POINTER(C.c_double)]
         L.oracle_norm2.restype = C.c_double
         L.oracle_norm2.argtypes = [vp, dp, C.c_int]
         L.oracle_cg.restype = C.c_int
@@ -108,22 +112,27 @@ class Oracle:
         self._L.oracle_hop(self._h, _p(U), kappa, _p(fin), _p(fout), par, sign)
         return fout
 
-    def meo(self, U, kappa, fin, sign):
-        """Dirac_Wilson_evenodd::apply (sign=+1) / dagger (sign=-1) on a full field."""
+    def meo(self, U, kappa, fin, sign, mh=0.0):
+        """Dirac_Wilson_evenodd::apply (sign=+1) / dagger (sign=-1) on a full field; with
+        mh != 0, Hasenbusch_operator::apply / dagger (M + mh)."""
         U, fin = self._links(U), self._field(fin)
         out = np.zeros_like(fin)
-        self._L.oracle_meo(self._h, _p(U), kappa, _p(fin), _p(out), sign)
+        if mh == 0.0:
+            self._L.oracle_meo(self._h, _p(U), kappa, _p(fin), _p(out), sign)
+        else:
+            self._L.oracle_hasenbusch(self._h, _p(U), kappa, mh, _p(fin), _p(out), sign)
         return out
 
     def norm2(self, f, par=EVEN):
         return float(self._L.oracle_norm2(self._h, _p(self._field(f)), par))
 
-    def cg(self, U, kappa, b, x0=None, accuracy=1e-12, maxiters=10000):
-        """CG<Dirac_Wilson_evenodd>(D, accuracy, maxiters).apply(b, x).
-        Returns (x, steps, rr_rel) with HILA's printed step count and relative residue."""
+    def cg(self, U, kappa, b, x0=None, accuracy=1e-12, maxiters=10000, mh=0.0):
+        """CG<Dirac_Wilson_evenodd>(D, accuracy, maxiters).apply(b, x); with mh != 0,
+        CG<Hasenbusch_operator<...>>.  Returns (x, steps, rr_rel) with HILA's printed step
+        count and relative residue."""
         U, b = self._links(U), self._field(b)
         x = np.zeros_like(b) if x0 is None else self._field(x0).copy()
         rr = C.c_double(0)
-        steps = self._L.oracle_cg(self._h, _p(U), kappa, _p(b), _p(x), accuracy, maxiters,
-                                C.byref(rr))
+        steps = self._L.oracle_cg_h(self._h, _p(U), kappa, mh, _p(b), _p(x), accuracy,
+                                    maxiters, C.byref(rr))
         return x, int(steps), float(rr.value)

@@ -388,6 +388,17 @@ void oracle_meo(const olat *l, const double *U, double kappa, const double *in, 
     zero_par(l, out, P_ODD);                            /* out[ODD] = 0 */
 }
 
+/* Hasenbusch_operator<Dirac_Wilson_evenodd>::apply / ::dagger (Hasenbusch.h:40-48):
+ * D.apply(in, out) (or dagger), then out[D.par] = out[X] + h_parameter * in[X]. */
+void oracle_hasenbusch(const olat *l, const double *U, double kappa, double mh, const double *in,
+                       double *out, int sign) {
+    oracle_meo(l, U, kappa, in, out, sign);
+    if (mh == 0.0) return;
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < l->Vh; ++s)
+        for (int k = 0; k < 24; ++k) out[24 * s + k] = out[24 * s + k] + mh * in[24 * s + k];
+}
+
 /* squarenorm summed over the sites of par (conjugate_gradient.h:61,72,79,87) */
 double oracle_norm2(const olat *l, const double *f, int par) {
     double sum = 0;
@@ -405,8 +416,19 @@ double oracle_norm2(const olat *l, const double *f, int par) {
 /* CG<Dirac_Wilson_evenodd>::apply(in, out), conjugate_gradient.h:46-104.  out is the initial
  * guess on entry.  Returns the loop counter i at exit (HILA's printed step count); *rr_rel
  * receives rrnew / source_norm as printed by conjugate_gradient.h:103. */
+int oracle_cg_h(const olat *l, const double *U, double kappa, double mh, const double *in,
+                double *out, double accuracy, int maxiters, double *rr_rel);
+
 int oracle_cg(const olat *l, const double *U, double kappa, const double *in, double *out,
               double accuracy, int maxiters, double *rr_rel) {
+    return oracle_cg_h(l, U, kappa, 0.0, in, out, accuracy, maxiters, rr_rel);
+}
+
+/* CG<Hasenbusch_operator<Dirac_Wilson_evenodd>>::apply (Hasenbusch.h:15-53 with
+ * conjugate_gradient.h:46-104): the same recurrence on (M + mh)^dag (M + mh); mh = 0 is
+ * CG<Dirac_Wilson_evenodd>. */
+int oracle_cg_h(const olat *l, const double *U, double kappa, double mh, const double *in,
+                double *out, double accuracy, int maxiters, double *rr_rel) {
     const int64_t n = 24 * l->V;
     double *r = (double *)calloc(n, sizeof(double));
     double *p = (double *)calloc(n, sizeof(double));
@@ -420,8 +442,8 @@ int oracle_cg(const olat *l, const double *U, double kappa, const double *in, do
     source_norm = oracle_norm2(l, in, par);
     target_rr = accuracy * accuracy * source_norm;
 
-    oracle_meo(l, U, kappa, out, Dp, +1);
-    oracle_meo(l, U, kappa, Dp, DDp, -1);
+    oracle_hasenbusch(l, U, kappa, mh, out, Dp, +1);
+    oracle_hasenbusch(l, U, kappa, mh, Dp, DDp, -1);
 #pragma omp parallel for schedule(static)
     for (int64_t s = 0; s < l->Vh; ++s)
         for (int k = 0; k < 24; ++k) {
@@ -432,8 +454,8 @@ int oracle_cg(const olat *l, const double *U, double kappa, const double *in, do
 
     for (i = 0; i < maxiters; i++) {
         pDp = rrnew = 0;
-        oracle_meo(l, U, kappa, p, Dp, +1);
-        oracle_meo(l, U, kappa, Dp, DDp, -1);
+        oracle_hasenbusch(l, U, kappa, mh, p, Dp, +1);
+        oracle_hasenbusch(l, U, kappa, mh, Dp, DDp, -1);
         pDp = oracle_norm2(l, Dp, par);
         alpha = rr / pDp;
 #pragma omp parallel for schedule(static)
