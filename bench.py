@@ -458,6 +458,72 @@ def config_io(W, lat, U, reps=3):
             "read_back_bit_exact": exact, "file": "temporary file, read from the page cache"}
 
 
+def hmc_solver_config(W, nbasis=4, nseq=7, mh=0.1, noise=1e-3, kappa=0.124, tol=1e-10):
+    """The solver steps of HILA's HMC fermion actions at 32^4 (fermion_field.h): CG on the
+    Hasenbusch operator M + mh, and a chronological sequence of solves (right-hand sides
+    c0 + k c1 + k^2 c2 + noise) each started from the MRE guess of the last `nbasis`
+    solutions (wl_mre_guess), against starting from zero.  Links exp(i 0.3 H) seed 3."""
+    dims = DIMS
+    lat = W.Lattice(dims)
+    lat.upload_gauge(W.gen_gauge(dims, "smooth", 0.3, 3))
+    D = W.Dirac_Wilson_evenodd(kappa, lat)
+    H = W.Hasenbusch_operator(D, mh)
+    src = [W.gen_spinor(dims, 500 + i, W.EVEN) for i in range(3)]
+    rng = np.random.default_rng(5)
+    b = W.Field(lat)
+    x = W.Field(lat)
+    out = {"lattice": list(dims), "kappa": kappa, "tolerance": tol}
+    # Hasenbusch-shifted CG
+    b.upload(src[0])
+    t = []
+    for _ in range(3):
+        x.zero()
+        t0 = time.perf_counter()
+        st = W.CG(H, tol).apply(b, x)[0]
+        t.append(time.perf_counter() - t0)
+    D_st = W.CG(D, tol).apply(b, x.zero())[0]
+    out["hasenbusch_cg"] = {"mh": mh, "ms": round(1e3 * min(t), 2), "iterations": st["iterations"],
+                            "true_resid": st["true_resid"], "unshifted_iterations": D_st["iterations"]}
+    # chronological solves with and without the MRE guess
+    basis = W.Field(lat, W.EVEN, W.F64, nbasis)
+    seq = []
+    for k in range(nseq):
+        h = src[0] + k * src[1] + k * k * src[2]
+        h[: lat.Vh] += noise * np.linalg.norm(h) / np.sqrt(lat.Vh * 24) * rng.standard_normal((lat.Vh, 24))
+        seq.append(h)
+    it_zero, it_mre, ms_zero, ms_mre, ms_guess = [], [], [], [], []
+    for k, h in enumerate(seq):
+        b.upload(h)
+        x.zero()
+        t0 = time.perf_counter()
+        st0 = W.CG(D, tol).apply(b, x)[0]
+        ms_zero.append(1e3 * (time.perf_counter() - t0))
+        it_zero.append(st0["iterations"])
+        n = min(k, nbasis)
+        t0 = time.perf_counter()
+        W.MRE_guess(x, b, D, basis, n)
+        g = 1e3 * (time.perf_counter() - t0)
+        st1 = W.CG(D, tol).apply(b, x)[0]
+        ms_mre.append(1e3 * (time.perf_counter() - t0))
+        ms_guess.append(g)
+        it_mre.append(st1["iterations"])
+        assert st1["converged"] == 1
+        # save_new_solution, newest first: shift the ring down, insert at 0
+        for j in range(min(k, nbasis - 1), 0, -1):
+            basis.copy_rhs_from(j, basis, j - 1)
+        basis.copy_rhs_from(0, x)
+    tail = slice(nbasis, nseq)
+    out["mre_chronological"] = {
+        "nbasis": nbasis, "sequence": nseq, "noise": noise,
+        "iterations_from_zero": it_zero, "iterations_from_guess": it_mre,
+        "guess_ms": round(float(np.median(ms_guess[1:])), 2),
+        "solve_ms_from_zero": round(float(np.median(ms_zero[tail])), 2),
+        "solve_ms_from_guess_incl_guess": round(float(np.median(ms_mre[tail])), 2),
+        "speedup": round(float(np.median(ms_zero[tail]) / np.median(ms_mre[tail])), 3)}
+    lat.close()
+    return out
+
+
 def cpu_baseline(U, psi, seconds=15.0):
     """The oracle's HILA-structured hopping term (projection pass + per-direction
     accumulation, wilson.h:53-95) on the host cores, on a bounded number of applications."""
@@ -498,6 +564,8 @@ def main(argv=None):
     ap.add_argument("--no-serve", action="store_true", help="skip the serving configuration")
     ap.add_argument("--no-config-io", action="store_true",
                     help="skip writing and reading the 32^4 configuration file")
+    ap.add_argument("--no-hmc", action="store_true",
+                    help="skip the Hasenbusch CG and MRE-guess configuration")
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--profile-multi", action="store_true",
                     help="short run for rocprofv3: --steps autotuned multi-rhs hopping launches "
@@ -597,6 +665,10 @@ def main(argv=None):
         multi = multi_rhs_config(W)
     if a.cg_solves > 0 and not a.no_serve:
         serve = serving_config(W, n_poisson=a.serve_requests)
+    hmc = None
+    if a.cg_solves > 0 and not a.no_hmc:
+        hmc = hmc_solver_config(W)
+        print(f"# hmc solvers: {hmc}", file=sys.stderr, flush=True)
 
     traffic = None
     if os.path.exists(PMC_FILE):
@@ -632,6 +704,7 @@ def main(argv=None):
         "multi_rhs_32x32x32x32": multi,
         "serving": serve,
         "config_file_32x32x32x32": cfg_io,
+        "hmc_solvers_32x32x32x32": hmc,
         "cpu_baseline": None,
     }
     if not a.no_cpu:

@@ -174,6 +174,7 @@ class Dirac_Wilson_evenodd {
     void apply(const WilsonField &in, WilsonField &out) { meo(in, out, +1); }
     void dagger(const WilsonField &in, WilsonField &out) { meo(in, out, -1); }
     GaugeField &gauge() { return gauge_; }
+    double shift() const { return 0.0; }  // the operator is M itself
 
   private:
     void meo(const WilsonField &in, WilsonField &out, int sign) {
@@ -183,6 +184,54 @@ class Dirac_Wilson_evenodd {
     }
     GaugeField &gauge_;
 };
+
+// Hasenbusch_operator<Dirac_type> (libraries/dirac/Hasenbusch.h:15-53): D + mh, whose apply /
+// dagger add h_parameter * in on D.par after D's own (here inside the same GPU pass).
+template <typename Dirac_type>
+class Hasenbusch_operator {
+  public:
+    using vector_type = typename Dirac_type::vector_type;
+    Dirac_type D;
+    Parity par;
+    double h_parameter;
+    double kappa;
+
+    Hasenbusch_operator(Dirac_type &d, double hasenbusch_parameter)
+        : D(d), par(d.par), h_parameter(hasenbusch_parameter), kappa(d.kappa) {}
+    Hasenbusch_operator(Hasenbusch_operator &h)
+        : D(h.D), par(h.par), h_parameter(h.h_parameter), kappa(h.kappa) {}
+
+    void apply(const vector_type &in, vector_type &out) { op(in, out, +1); }
+    void dagger(const vector_type &in, vector_type &out) { op(in, out, -1); }
+    GaugeField &gauge() { return D.gauge(); }
+    double shift() const { return h_parameter; }
+
+  private:
+    void op(const vector_type &in, vector_type &out, int sign) {
+        check(wl_hasenbusch_apply(gauge().lattice().ctx(), out.part(EVEN), in.part(EVEN), kappa,
+                                  h_parameter, sign),
+              "wl_hasenbusch_apply");
+        out.zero(ODD);
+    }
+};
+
+// MRE_guess(psi, chi, D, old_chi_inv) (libraries/hmc/MRE_guess.h:8-87): psi = the
+// minimal-residual combination of the previous solutions old_chi_inv (at most 8) for
+// D^dag D psi = chi, computed on the GPU (wl_mre_guess); psi's odd half is set to 0.
+template <typename DIRAC_OP>
+void MRE_guess(WilsonField &psi, const WilsonField &chi, DIRAC_OP &D,
+               const std::vector<WilsonField> &old_chi_inv) {
+    wl_ctx *c = D.gauge().lattice().ctx();
+    const int n = (int)old_chi_inv.size();
+    wl_field *basis = nullptr;
+    check(wl_field_create(c, WL_EVEN, WL_F64, n > 0 ? n : 1, &basis), "wl_field_create");
+    int rc = WL_OK;
+    for (int i = 0; i < n && rc == WL_OK; ++i) rc = wl_field_copy_rhs(basis, i, old_chi_inv[i].part(EVEN), 0);
+    if (rc == WL_OK) rc = wl_mre_guess(c, psi.part(EVEN), chi.part(EVEN), basis, n, D.kappa, D.shift());
+    wl_field_destroy(basis);
+    check(rc, "MRE_guess");
+    psi.zero(ODD);
+}
 
 // CG<Op> (conjugate_gradient.h:19-105): solves (M^dag M) out = in on M.par.
 template <typename Op>
@@ -197,13 +246,16 @@ class CG {
     // HILA); the statistics of the last call are kept in `stats`.
     void apply(vector_type &in, vector_type &out) {
         wl_ctx *c = M.gauge().lattice().ctx();
-        const int rc = inner_prec_ == WL_F64
-                           ? wl_cg_solve(c, out.part(EVEN), in.part(EVEN), M.kappa, accuracy,
-                                         (int)maxiters, &stats)
-                           : wl_cg_solve_mixed(c, out.part(EVEN), in.part(EVEN), M.kappa, accuracy,
-                                               (int)maxiters, inner_prec_, delta_, update_every_,
-                                               &stats);
-        if (rc != WL_E_NOCONV) check(rc, "wl_cg_solve");
+        wl_cg_params p{};
+        p.kappa = M.kappa;
+        p.mh = M.shift();  // Hasenbusch_operator: (M + mh)^dag (M + mh)
+        p.accuracy = accuracy;
+        p.maxiters = (int)maxiters;
+        p.inner_prec = inner_prec_;
+        p.delta = delta_;
+        p.update_every = update_every_;
+        const int rc = wl_cg_solve_ex(c, out.part(EVEN), in.part(EVEN), &p, &stats);
+        if (rc != WL_E_NOCONV) check(rc, "wl_cg_solve_ex");
     }
     // Mixed precision (not in HILA's CG): inner operator and Krylov vectors in WL_F32 or WL_F16
     // (with per-site scale), FP64 reliable updates when ||r|| drops by delta or every

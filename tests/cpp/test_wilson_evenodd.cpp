@@ -135,11 +135,70 @@ static void run(Lattice &lattice, int link_kind, const char *name) {
     CHECK(Db.squarenorm(ODD) == 0.0, "out[ODD] = 0");
 }
 
+// test_CG.cpp:290-345 (the Hasenbusch_operator block), then MRE_guess (MRE_guess.h) on a
+// chronological sequence of solves: with the three previous solutions of right-hand sides
+// quadratic in k, the guess for the next one leaves CG little to do.
+static void run_hasenbusch(Lattice &lattice) {
+    std::printf("Checking with Hasenbusch_operator\n");
+    GaugeField U(lattice);
+    U.random(1, 0.0, 9);
+    using dirac_base = Dirac_Wilson_evenodd<SU3d>;
+    dirac_base Dbase(0.1, U);
+    using dirac = Hasenbusch_operator<dirac_base>;
+    dirac D(Dbase, 0.1);
+    WilsonField a(lattice), b(lattice), Db(lattice), Ddaggera(lattice), DdaggerDb(lattice);
+    a.zero(ODD);
+    b.zero(ODD);
+    a.gaussian_random(21, EVEN);
+    b.gaussian_random(22, EVEN);
+    D.apply(b, Db);
+    D.dagger(a, Ddaggera);
+    const std::complex<double> d = dot_even(a, Db) - dot_even(Ddaggera, b);
+    CHECK(std::norm(d) < 1e-26 * std::norm(dot_even(a, Db)), "test Hasenbusch_operator (adjoint)");
+    // b = 1/D a -> Db = a
+    CG<dirac> inverse(D);
+    b.zero(ALL);
+    inverse.apply(Ddaggera, b);
+    D.apply(b, Db);
+    CHECK(diff2_even(a, Db) < 1e-16 * a.squarenorm(EVEN), "test CG (Hasenbusch)");
+    // DdgD (DdgD)^-1 a = a
+    a.gaussian_random(23, EVEN);
+    b.zero(ALL);
+    inverse.apply(a, b);
+    D.apply(b, Db);
+    D.dagger(Db, DdaggerDb);
+    CHECK(diff2_even(a, DdaggerDb) < 1e-16 * a.squarenorm(EVEN), "test DdgD (DdgD)^-1 (Hasenbusch)");
+
+    // chronological MRE guesses: chi_k = c0 + k c1 + k^2 c2
+    WilsonField c0(lattice), c1(lattice), c2(lattice);
+    c0.gaussian_random(24, EVEN);
+    c1.gaussian_random(25, EVEN);
+    c2.gaussian_random(26, EVEN);
+    const std::vector<double> h0 = c0.host(), h1 = c1.host(), h2 = c2.host();
+    std::vector<WilsonField> old_chi_inv;
+    int iters[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k) {
+        std::vector<double> h(h0.size());
+        for (size_t i = 0; i < h.size(); ++i) h[i] = h0[i] + k * h1[i] + k * k * h2[i];
+        WilsonField chi(lattice), psi(lattice);
+        chi.set_host(h);
+        psi.zero(ALL);
+        MRE_guess(psi, chi, D, old_chi_inv);
+        CG<dirac> solve(D, 1e-10);
+        solve.apply(chi, psi);
+        iters[k] = solve.stats.iterations;
+        old_chi_inv.insert(old_chi_inv.begin(), psi);  // save_new_solution, newest first
+    }
+    std::printf("      CG iterations from the MRE guess: %d %d %d %d\n", iters[0], iters[1], iters[2], iters[3]);
+    CHECK(iters[3] * 4 <= iters[0], "MRE guess from three previous solutions");
+}
+
 int main() {
     try {
         Lattice lattice({16, 8, 8, 8});  // test_CG.cpp:57, NDIM == 4
         run(lattice, 0, "unit");
         run(lattice, 1, "Haar random");
+        run_hasenbusch(lattice);
     } catch (const error &e) {
         std::printf("FAIL: exception %s\n", e.what());
         return 2;

@@ -180,3 +180,39 @@ def test_mre_chronological_guess_cuts_iterations(setup8):
         iters.append(W.CG(D, 1e-10).apply(chi, x)[0]["iterations"])
         B.copy_rhs_from(k, x)
     assert iters[0] > 20 and iters[3] * 4 <= iters[0], iters
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_served_shifted_request_matches_blocking_solver(precision):
+    """A request with mh is solved with M + mh: the same bits as the blocking batched solver on
+    the Hasenbusch operator, and the FP64 true residual of the shifted system meets the
+    tolerance."""
+    dims, nrhs, kappa, tol = (8, 8, 8, 8), 3, 0.124, 1e-10
+    U = W.gen_gauge(dims, "smooth", 0.3, 430)
+    Vh = int(np.prod(dims)) // 2
+    src = np.ascontiguousarray(np.stack([_even(dims, 431 + j)[:Vh] for j in range(nrhs)]))
+    sol = np.zeros_like(src)
+    _say("served shifted request", precision)
+    with W.Server() as S:
+        t = S.submit(dims, U, src, sol, kappa, tol, precision=precision, delta=0.1, mh=MH)
+        res, st = S.wait(t)
+    assert res["status"] == W.WL_OK
+    lat = W.Lattice(dims)
+    lat.upload_gauge(U)
+    H = W.Hasenbusch_operator(W.Dirac_Wilson_evenodd(kappa, lat), MH)
+    b = W.Field(lat, EVEN, W.F64, nrhs)
+    x = W.Field(lat, EVEN, W.F64, nrhs)
+    for j in range(nrhs):
+        b.upload(src[j], rhs=j)
+    bst = W.CG(H, tol, precision=precision, delta=0.1, update_every=100).apply(b, x)
+    orc = Oracle(dims)
+    for j in range(nrhs):
+        assert np.array_equal(x.download(j), sol[j]), j
+        assert bst[j]["iterations"] == st[j]["iterations"]
+        full = np.zeros((2 * Vh, 24))
+        full[:Vh] = sol[j]
+        sf = np.zeros_like(full)
+        sf[:Vh] = src[j]
+        r = sf - orc.meo(U, kappa, orc.meo(U, kappa, full, 1, mh=MH), -1, mh=MH)
+        assert np.linalg.norm(r) <= 1.5 * tol * np.linalg.norm(sf)
+    lat.close()
