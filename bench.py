@@ -420,6 +420,26 @@ def profile_multi(W, steps):
     lat.close()
 
 
+def profile_mre(W, steps):
+    """MRE guesses at 32^4 (links exp(i 0.3 H) seed 3, kappa 0.124) from 4 and from 8 Gaussian
+    basis vectors, `steps` times each: the multi-rhs M^dag M on the basis, the inner-product
+    pass (k_mre_dots) and the combination (k_mre_combine)."""
+    dims = DIMS
+    lat = W.Lattice(dims)
+    lat.upload_gauge(W.gen_gauge(dims, "smooth", 0.3, 3))
+    D = W.Dirac_Wilson_evenodd(0.124, lat)
+    chi = W.Field(lat).upload(W.gen_spinor(dims, 600, W.EVEN))
+    psi = W.Field(lat)
+    for n in (4, 8):
+        B = W.Field(lat, W.EVEN, W.F64, n)
+        for j in range(n):
+            B.upload(W.gen_spinor(dims, 610 + j, W.EVEN), rhs=j)
+        for _ in range(steps):
+            W.MRE_guess(psi, chi, D, B, n)
+        print(f"# profile-mre: {steps} guesses from {n} vectors", file=sys.stderr, flush=True)
+    lat.close()
+
+
 def profile_serve(W):
     """The closed batch of the serving configuration (24 requests), FP16 inner."""
     rng = np.random.default_rng(11)
@@ -571,6 +591,8 @@ def main(argv=None):
                     help="short run for rocprofv3: --steps autotuned multi-rhs hopping launches "
                          "(12 rhs, FP16 then FP32 storage) at 32^4, then --steps copies of one "
                          "12-rhs FP16 field; prints no JSON line")
+    ap.add_argument("--profile-mre", action="store_true",
+                    help="short run for rocprofv3: MRE guesses at 32^4 from 4 and 8 vectors")
     ap.add_argument("--profile-serve", action="store_true",
                     help="short run for rocprofv3: the closed batch of 24 served requests, FP16 "
                          "inner; prints no JSON line")
@@ -591,6 +613,9 @@ def main(argv=None):
         return
     if a.profile_serve:
         profile_serve(W)
+        return
+    if a.profile_mre:
+        profile_mre(W, a.steps)
         return
 
     t_setup = time.perf_counter()
