@@ -23,7 +23,7 @@ LIBDIR = os.path.join(ROOT, "hila_amd", "lib")
 LIB = os.path.join(LIBDIR, "libhila_wilson.so")
 
 HIP_SOURCES = ["wl_rb_f16.hip", "wl_rb_f32.hip", "wl_rb_f64.hip", "wl_dslash.hip", "wl_multi.hip",
-               "wl_blas.hip", "wl_api.hip", "wl_cg.hip", "wl_serve.hip", "wl_config.hip", "wl_mre.hip"]
+               "wl_blas.hip", "wl_api.hip", "wl_cg.hip", "wl_serve.hip", "wl_config.hip", "wl_mre.hip", "wl_force.hip"]
 # C++ programs written against the HILA-style mirror (include/hila_wilson.hpp)
 CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "test_wilson_evenodd.cpp")]
 CPP_TEST_BUILD = os.path.join(ROOT, "tests", "cpp", "_build")

@@ -449,7 +449,7 @@ extern "C" int wl_destroy(wl_ctx *c) {
     }
     c->user_fields.clear();
     for (void *p : {(void *)c->st_mr, (void *)c->flags_d, (void *)c->gauge64, c->gauge32, c->gauge16,
-                    c->staging, (void *)c->partials, (void *)c->st, (void *)c->scratch_d})
+                    c->staging, (void *)c->partials, (void *)c->st, (void *)c->scratch_d, c->force64, c->mom64})
         if (p) (void)hipFree(p);
     if (c->st_host) (void)hipHostFree(c->st_host);
     if (c->scratch_h) (void)hipHostFree(c->scratch_h);

@@ -189,6 +189,13 @@ int ensure_gauge64(wl_ctx *c) {
 
 }  // namespace
 
+hipError_t launch_links_to_host(const Geom &g, const void *planes, int mu, double *out, int lexicographic,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(k_gauge_to_host, site_grid(g), dim3(kBlock), 0, s, (const cx<double> *)planes, (d2 *)out, mu,
+                       g, lexicographic);
+    return hipGetLastError();
+}
+
 int config_upload(wl_ctx *c, const char *path) {
     File f;
     int insize[4];

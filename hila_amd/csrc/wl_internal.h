@@ -64,6 +64,15 @@ hipError_t launch_gauge_to_soa(const Geom &g, int mu, const double *aos_full, vo
                                hipStream_t s);
 hipError_t launch_gauge_convert(const Geom &g, int prec, const void *gauge64, void *gauge_lo,
                                 hipStream_t s);
+// direction mu of link planes (gauge layout, FP64) -> 18 doubles per site, sites in
+// lexicographic order (lexicographic != 0) or EVEN_SITES_FIRST order (wl_config.hip)
+hipError_t launch_links_to_host(const Geom &g, const void *planes, int mu, double *out, int lexicographic,
+                                hipStream_t s);
+// fermion force (wl_force.hip): F of Dirac_Wilson_evenodd::force from chi, psi (even) and
+// tA = -kappa H_{-sign} chi, tB = -kappa H_sign psi (odd), into `out` planes (momentum == 0),
+// or out += project_antihermitean(U coef F) (momentum != 0)
+hipError_t launch_force(const Geom &g, int sign, int momentum, const void *chi, const void *psi, const void *tA,
+                        const void *tB, const void *gauge, void *out, double kappa, double coef, hipStream_t s);
 // sum over sites of ||f_j||^2 for rhs j of an N-rhs field (FP64 accumulate), then `fin`
 hipError_t launch_norm2(int prec, const Geom &g, const void *f, int N, int j, const RedArgs &ra,
                         int fin, const int *done, hipStream_t s);
@@ -161,6 +170,8 @@ struct wl_ctx {
     void *gauge64 = nullptr;
     void *gauge32 = nullptr;
     void *gauge16 = nullptr;
+    void *force64 = nullptr;           // fermion force planes (gauge layout), on first use
+    void *mom64 = nullptr;             // HMC momentum planes (gauge layout), zero until uploaded
     bool gauge_ok = false;
     bool gauge32_ok = false;
     bool gauge16_ok = false;

@@ -40,6 +40,7 @@ EXPORTED_SYMBOLS = [
     "wl_host_alloc", "wl_host_free", "wl_autotune", "wl_server_set_option",
     "wl_gauge_download", "wl_config_header", "wl_config_read", "wl_config_write",
     "wl_hasenbusch_apply", "wl_cg_solve_ex", "wl_mre_guess", "wl_field_copy_rhs",
+    "wl_fermion_force", "wl_momentum_upload", "wl_momentum_download", "wl_fermion_force_step",
 ]
 
 
@@ -154,6 +155,10 @@ def lib():
         L.wl_cg_solve_ex.argtypes = [vp, vp, vp, C.POINTER(CGParams), C.POINTER(CGStats)]
         L.wl_mre_guess.argtypes = [vp, vp, vp, vp, i, d, d]
         L.wl_field_copy_rhs.argtypes = [vp, i, vp, i]
+        L.wl_fermion_force.argtypes = [vp, vp, vp, d, i, C.POINTER(vp)]
+        L.wl_momentum_upload.argtypes = [vp, C.POINTER(vp)]
+        L.wl_momentum_download.argtypes = [vp, C.POINTER(vp)]
+        L.wl_fermion_force_step.argtypes = [vp, vp, vp, d, C.POINTER(CGParams), C.POINTER(CGStats)]
         for name in EXPORTED_SYMBOLS:
             if name not in ("wl_version", "wl_last_error", "wl_volume"):
                 getattr(L, name).restype = C.c_int
@@ -242,6 +247,23 @@ class Lattice:
         ptrs = (C.c_void_p * 4)(*[U[m].ctypes.data for m in range(4)])
         check(lib().wl_gauge_download(self.handle, ptrs))
         return U
+
+    def _links_in(self, P, what):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        if P.shape != (4, self.V, 18):
+            raise ValueError(f"{what} must have shape (4, {self.V}, 18)")
+        return P, (C.c_void_p * 4)(*[P[m].ctypes.data for m in range(4)])
+
+    def upload_momentum(self, P):
+        """The HMC momentum (4, V, 18), HILA layout (wl_momentum_upload)."""
+        P, ptrs = self._links_in(P, "momentum")
+        check(lib().wl_momentum_upload(self.handle, ptrs))
+
+    def download_momentum(self):
+        P = np.empty((4, self.V, 18), dtype=np.float64)
+        ptrs = (C.c_void_p * 4)(*[P[m].ctypes.data for m in range(4)])
+        check(lib().wl_momentum_download(self.handle, ptrs))
+        return P
 
     def config_read(self, path):
         """GaugeField::config_read (gaugefield.h:219-272): the gauge field from a HILA
@@ -426,6 +448,29 @@ def MRE_guess(psi: Field, chi: Field, D, basis: Field, nbasis=None):
     check(lib().wl_mre_guess(D.lat.handle, psi.handle, chi.handle, basis.handle, n, D.kappa,
                              getattr(D, "mh", 0.0)))
     return psi
+
+
+def fermion_force(D, chi: Field, psi: Field, sign=1):
+    """D.force(chi, psi, force, sign) (wilson.h:294-311): (4, V, 18) float64, HILA layout; the
+    derivative of Re <chi, M_sign psi> with respect to the links."""
+    F = np.empty((4, D.lat.V, 18), dtype=np.float64)
+    ptrs = (C.c_void_p * 4)(*[F[m].ctypes.data for m in range(4)])
+    check(lib().wl_fermion_force(D.lat.handle, chi.handle, psi.handle, D.kappa, int(sign), ptrs))
+    return F
+
+
+def force_step(D, chi: Field, psi: Field, eps, accuracy=1e-12, maxiters=10000, precision="f64",
+               delta=0.1, update_every=100):
+    """fermion_action::force_step(eps) (fermion_field.h:146-167) on the momentum the lattice
+    holds: psi (initial guess in, CG solution out), then momentum += P(U force).  Returns the
+    CG stats dict."""
+    p = CGParams(kappa=D.kappa, mh=getattr(D, "mh", 0.0), accuracy=float(accuracy),
+                 maxiters=int(maxiters), inner_prec={"f64": F64, "f32": F32, "f16": F16}[precision],
+                 delta=float(delta), update_every=int(update_every))
+    st = CGStats()
+    check(lib().wl_fermion_force_step(D.lat.handle, chi.handle, psi.handle, float(eps), C.byref(p),
+                                      C.byref(st)), allow=(WL_E_NOCONV,))
+    return st.as_dict()
 
 
 class CG:

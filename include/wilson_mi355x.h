@@ -230,6 +230,34 @@ int wl_cg_solve_ex(wl_ctx *ctx, wl_field *x, const wl_field *b, const wl_cg_para
 int wl_mre_guess(wl_ctx *ctx, wl_field *psi, const wl_field *chi, const wl_field *basis, int nbasis,
                  double kappa, double mh);
 
+/* ---- fermion force and HMC momentum ------------------------------------------------------ */
+
+/* Dirac_Wilson_evenodd::force(chi, psi, force, sign) (libraries/dirac/wilson.h:294-311, with
+ * Dirac_Wilson_calc_force :108-137): the derivative of Re <chi, M_sign psi> with respect to
+ * the links, F[mu] in HILA's layout of a Field<SquareMatrix<3,Complex<double>>> (V x 18
+ * doubles, EVEN_SITES_FIRST, row-major): for every link, d/de Re <chi, M_sign(U + e E) psi>
+ * = Re tr(E F_mu(x)) when U_mu(x) alone moves to U_mu(x) + e E.  chi and psi are even FP64
+ * fields of one right-hand side.  The same for Hasenbusch_operator (its force is D's). */
+int wl_fermion_force(wl_ctx *ctx, const wl_field *chi, const wl_field *psi, double kappa, int sign,
+                     double *const F[4]);
+
+/* The HMC momentum the context holds (gauge_field_base::momentum, hmc/gauge_field.h:62; one
+ * 3x3 complex matrix per link), in HILA's layout as the gauge field.  Zero until uploaded. */
+int wl_momentum_upload(wl_ctx *ctx, const double *const P[4]);
+int wl_momentum_download(wl_ctx *ctx, double *const P[4]);
+
+/* fermion_action::force_step(eps) (libraries/hmc/fermion_field.h:146-167) for the operator of
+ * *params (Dirac_Wilson_evenodd, or its Hasenbusch_operator when params->mh != 0):
+ *   psi = CG solution of (D^dag D) psi = chi, from the initial guess psi holds on entry
+ *         (e.g. a wl_mre_guess); psi holds it on exit;
+ *   force = -eps (D.force(D psi, psi, +1) + D.force(psi, D psi, -1));
+ *   momentum += project_antihermitean(U force)  (gauge_field::add_momentum, gauge_field.h:249-257),
+ * where project_antihermitean(M) = (M - M^dag)/2 - tr((M - M^dag)/2)/3.  The force and the
+ * momentum update are one pass over the links.  Returns WL_E_NOCONV (after completing the
+ * step, as HILA's CG returns silently) when CG ran out of iterations; stats as wl_cg_solve. */
+int wl_fermion_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, double eps,
+                          const wl_cg_params *params, wl_cg_stats *stats);
+
 /* ---- kernel options ------------------------------------------------------------------- */
 
 /* Tuning knobs of the hopping kernel (no counterpart in HILA, whose GPU loops are generated

@@ -48,6 +48,8 @@ def lib(variant=0):
             getattr(L, fn).argtypes = [vp, dp, C.c_double, dp, dp, C.c_int, C.c_int]
         L.oracle_meo.argtypes = [vp, dp, C.c_double, dp, dp, C.c_int]
         L.oracle_hasenbusch.argtypes = [vp, dp, C.c_double, C.c_double, dp, dp, C.c_int]
+        L.oracle_calc_force.argtypes = [vp, C.c_double, dp, dp, dp, C.c_int, C.c_int]
+        L.oracle_force.argtypes = [vp, dp, C.c_double, dp, dp, dp, C.c_int]
         L.oracle_cg_h.restype = C.c_int
         L.oracle_cg_h.argtypes = [vp, dp, C.c_double, C.c_double, dp, dp, C.c_double, C.c_int,
                                   C.POINTER(C.c_double)]
@@ -121,6 +123,20 @@ class Oracle:
             self._L.oracle_meo(self._h, _p(U), kappa, _p(fin), _p(out), sign)
         else:
             self._L.oracle_hasenbusch(self._h, _p(U), kappa, mh, _p(fin), _p(out), sign)
+        return out
+
+    def calc_force(self, kappa, chi, psi, par, sign):
+        """Dirac_Wilson_calc_force: (4, V, 18) float64, every site set."""
+        chi, psi = self._field(chi), self._field(psi)
+        out = np.zeros((4, self.V, 18))
+        self._L.oracle_calc_force(self._h, kappa, _p(chi), _p(psi), _p(out), par, sign)
+        return out
+
+    def force(self, U, kappa, chi, psi, sign):
+        """Dirac_Wilson_evenodd::force(chi, psi, force, sign): (4, V, 18) float64."""
+        U, chi, psi = self._links(U), self._field(chi), self._field(psi)
+        out = np.zeros((4, self.V, 18))
+        self._L.oracle_force(self._h, _p(U), kappa, _p(chi), _p(psi), _p(out), sign)
         return out
 
     def norm2(self, f, par=EVEN):

@@ -480,6 +480,70 @@ int oracle_cg_h(const olat *l, const double *U, double kappa, double mh, const d
     return i;
 }
 
+/* ------------------------------------------------------------- fermion force (wilson.h) */
+
+/* a.outer_product(b) summed over spin (wilson_vector.h:229-235, matrix.h:1437-1448):
+ * M(i, j) = sum_s a[s][i] conj(b[s][j]), scaled by f */
+static su3 wouter(const wvec *a, const wvec *b, double f) {
+    su3 m;
+    for (int i = 0; i < NC; ++i)
+        for (int j = 0; j < NC; ++j) {
+            cplx acc = {0, 0};
+            for (int q = 0; q < 4; ++q) acc = cadd(acc, cmul(a->c[q][i], cconj(b->c[q][j])));
+            m.e[i][j] = cscale(f, acc);
+        }
+    return m;
+}
+
+/* Dirac_Wilson_calc_force (wilson.h:108-137).  For every direction, with
+ * vtemp[0] = HalfWilsonVector(chi, dir, -sign) on opp(par) and vtemp[1] =
+ * HalfWilsonVector(psi, dir, sign) on par, both gathered with chi's boundary condition:
+ *   out[dir][par] = -kappa * vtemp[0][X+dir].expand(dir, -sign).outer_product(psi[X])
+ *   out[dir][opp] = -kappa * vtemp[1][X+dir].expand(dir,  sign).outer_product(chi[X])
+ * out: 4 x V x 18 doubles (SquareMatrix<3, Complex<double>>, row-major), every site set. */
+void oracle_calc_force(const olat *l, double kappa, const double *chi, const double *psi, double *out,
+                       int par, int sign) {
+    for (int dir = 0; dir < NDIM; ++dir) {
+#pragma omp parallel for schedule(static)
+        for (int64_t s = 0; s < l->V; ++s) {
+            const int64_t sn = l->nb[dir][s];
+            const double g = (double)l->sg[dir][s];
+            su3 m;
+            if (in_parity(l, s, par)) {
+                hvec h = half_project(W(chi, sn), dir, -sign);
+                h = h_scale(g, &h);
+                const wvec e = half_expand(&h, dir, -sign);
+                m = wouter(&e, W(psi, s), -kappa);
+            } else {
+                hvec h = half_project(W(psi, sn), dir, sign);
+                h = h_scale(g, &h);
+                const wvec e = half_expand(&h, dir, sign);
+                m = wouter(&e, W(chi, s), -kappa);
+            }
+            memcpy(out + 18 * ((int64_t)dir * l->V + s), &m, sizeof(su3));
+        }
+    }
+}
+
+/* Dirac_Wilson_evenodd::force(chi, psi, force, sign) (wilson.h:294-311): chi, psi full fields
+ * (their even halves are used); out as oracle_calc_force. */
+void oracle_force(const olat *l, const double *U, double kappa, const double *chi, const double *psi,
+                  double *out, int sign) {
+    const int64_t n = 24 * l->V;
+    double *tmp = (double *)calloc(n, sizeof(double));
+    double *f2 = (double *)malloc(sizeof(double) * 18 * 4 * l->V);
+    oracle_hop_set(l, U, kappa, chi, tmp, P_ODD, -sign);
+    oracle_calc_force(l, -kappa, tmp, psi, out, P_EVEN, sign);
+    memset(tmp, 0, n * sizeof(double));
+    oracle_hop_set(l, U, kappa, psi, tmp, P_ODD, sign);
+    oracle_calc_force(l, -kappa, chi, tmp, f2, P_ODD, sign);
+    const int64_t m = 18 * 4 * l->V;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m; ++i) out[i] = out[i] + f2[i];
+    free(tmp);
+    free(f2);
+}
+
 /* ---------------------------------------------------------- fused CPU variant (baseline) */
 
 /* The same D_oe / D_eo half operator written as one pass per output site (gather, project,
