@@ -440,6 +440,25 @@ def profile_mre(W, steps):
     lat.close()
 
 
+def profile_force(W, steps):
+    """Force steps at 32^4 (links exp(i 0.3 H) seed 3, kappa 0.124) from a solved psi, `steps`
+    times, then `steps` force evaluations with the force written out: the hopping launches
+    for the odd temporaries and the force kernel in its two modes."""
+    dims = DIMS
+    lat = W.Lattice(dims)
+    lat.upload_gauge(W.gen_gauge(dims, "smooth", 0.3, 3))
+    D = W.Dirac_Wilson_evenodd(0.124, lat)
+    chi = W.Field(lat).upload(W.gen_spinor(dims, 620, W.EVEN))
+    psi = W.Field(lat)
+    W.CG(D, 1e-12).apply(chi, psi)
+    for _ in range(steps):
+        W.force_step(D, chi, psi, 0.01, accuracy=1e-10)
+    for _ in range(steps):
+        W.fermion_force(D, chi, psi, 1)
+    print(f"# profile-force: {steps} force steps and {steps} forces", file=sys.stderr, flush=True)
+    lat.close()
+
+
 def profile_serve(W):
     """The closed batch of the serving configuration (24 requests), FP16 inner."""
     rng = np.random.default_rng(11)
@@ -532,6 +551,18 @@ def hmc_solver_config(W, nbasis=4, nseq=7, mh=0.1, noise=1e-3, kappa=0.124, tol=
         for j in range(min(k, nbasis - 1), 0, -1):
             basis.copy_rhs_from(j, basis, j - 1)
         basis.copy_rhs_from(0, x)
+    # fermion_action::force_step with psi already solved (CG stops at once): the force pass
+    mom = W.Field(lat)
+    b.upload(seq[-1])
+    chi = W.Field(lat).upload(seq[-1])
+    W.CG(D, 1e-12).apply(chi, x.zero())
+    fs = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        st = W.force_step(D, chi, x, 0.01, accuracy=1e-10)
+        fs.append(1e3 * (time.perf_counter() - t0))
+    out["force_step"] = {"ms_with_solved_psi": round(min(fs), 3), "cg_iterations": st["iterations"]}
+    del mom
     tail = slice(nbasis, nseq)
     out["mre_chronological"] = {
         "nbasis": nbasis, "sequence": nseq, "noise": noise,
@@ -593,6 +624,8 @@ def main(argv=None):
                          "12-rhs FP16 field; prints no JSON line")
     ap.add_argument("--profile-mre", action="store_true",
                     help="short run for rocprofv3: MRE guesses at 32^4 from 4 and 8 vectors")
+    ap.add_argument("--profile-force", action="store_true",
+                    help="short run for rocprofv3: force steps and forces at 32^4")
     ap.add_argument("--profile-serve", action="store_true",
                     help="short run for rocprofv3: the closed batch of 24 served requests, FP16 "
                          "inner; prints no JSON line")
@@ -616,6 +649,9 @@ def main(argv=None):
         return
     if a.profile_mre:
         profile_mre(W, a.steps)
+        return
+    if a.profile_force:
+        profile_force(W, a.steps)
         return
 
     t_setup = time.perf_counter()

@@ -36,72 +36,75 @@ struct ForceArgs {
     Geom g;
 };
 
-// acc += sum_t h_a[t] (x) h_b[t]^dag, h = upper rows of (1 + S g_MU) .
-template <int MU, int S>
-__device__ __forceinline__ void outer_proj(cx<double> (&acc)[3][3], const cx<double> (&a)[4][3],
-                                           const cx<double> (&b)[4][3], double bc) {
+// acc += ((1 + SIG g_MU) a) (x) b^dag = sum_t h_a[t] (x) h_b[t]^dag, h = the upper rows of
+// (1 + SIG g_MU) ., with a the spinor at site ka of `na` (times bc) and b the one at kb of
+// `nb`.  Each spinor is reduced to its half vector as soon as it is loaded.
+template <int MU, int SIG>
+__device__ __forceinline__ void force_term(cx<double> (&acc)[3][3], const cx<double> *na, int ka,
+                                           const cx<double> *nb, int kb, int S_, double bc) {
     cx<double> ha[2][3], hb[2][3];
-    project<MU, S>(a, ha);
-    project<MU, S>(b, hb);
+    {
+        cx<double> w[4][3];
+        ld_site<double>(na, S_, ka, w);
+        project<MU, SIG>(w, ha);
+    }
+    {
+        cx<double> w[4][3];
+        ld_site<double>(nb, S_, kb, w);
+        project<MU, SIG>(w, hb);
+    }
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j)  // a_i conj(b_j) = cmulc(b_j, a_i)
+            for (int j = 0; j < 3; ++j)  // a_i conj(b_j)
                 acc[i][j] = cfma(acc[i][j], bc * ha[t][i], cx<double>{hb[t][j].x, -hb[t][j].y});
 }
 
+// F for link (X, MU) of a site of parity p at checkerboard index k, forward neighbour kn:
+//   even X:  (1 - s g) tA(X+mu) (x) psi(X)^dag + (1 + s g) tB(X+mu) (x) chi(X)^dag
+//   odd X:   (1 + s g) psi(X+mu) (x) tA(X)^dag + (1 - s g) chi(X+mu) (x) tB(X)^dag
 template <int MU, int S>
-__device__ __forceinline__ void link_force(cx<double> (&F)[3][3], const cx<double> (&n1)[4][3],
-                                           const cx<double> (&o1)[4][3], const cx<double> (&n2)[4][3],
-                                           const cx<double> (&o2)[4][3], bool even, double bc) {
-    if (even) {
-        outer_proj<MU, -S>(F, n1, o1, bc);
-        outer_proj<MU, S>(F, n2, o2, bc);
+__device__ __forceinline__ void link_force(cx<double> (&F)[3][3], const ForceArgs &A, int p, int k, int kn,
+                                           double bc) {
+    const int S_ = A.g.stride;
+    if (p == 0) {
+        force_term<MU, -S>(F, A.tA, kn, A.psi, k, S_, bc);
+        force_term<MU, S>(F, A.tB, kn, A.chi, k, S_, bc);
     } else {
-        outer_proj<MU, S>(F, n1, o1, bc);
-        outer_proj<MU, -S>(F, n2, o2, bc);
+        force_term<MU, S>(F, A.psi, kn, A.tA, k, S_, bc);
+        force_term<MU, -S>(F, A.chi, kn, A.tB, k, S_, bc);
     }
 }
 
-// One thread per (checkerboard site, direction = blockIdx.y).
+// A workgroup takes one 64-site segment of the checkerboard (both parities in sequence: sites
+// s < vh even, s >= vh odd), its four waves the four directions, so that the site's own spinors
+// are fetched from memory once and served from the caches to the other three waves; segments
+// are spread over the XCDs in contiguous runs (xcd_tile) so that neighbouring segments share an
+// L2.  (Alternating the parities segment by segment measured no faster.)
 template <int S, int MODE>
 __global__ __launch_bounds__(kBlock) void k_force(ForceArgs A) {
-    const int s = blockIdx.x * kBlock + threadIdx.x;
+    const int s = xcd_tile(blockIdx.x, gridDim.x) * 64 + (int)(threadIdx.x & 63);
     const Geom &g = A.g;
     if (s >= 2 * g.vh) return;
-    const int mu = blockIdx.y;
+    const int mu = (int)(threadIdx.x >> 6);
     const int p = s >= g.vh;
     const int k = s - p * g.vh;
     const SiteNb nb = neighbours(g, k, p);
     const int kn = nb.fwd[mu];
     const double bc = (mu == 3 && g.antiperiodic && nb.fwd_edge_t) ? -1.0 : 1.0;
     const int S_ = g.stride;
-    // even X: (n1, o1) = (tA(X+mu), psi(X)), (n2, o2) = (tB(X+mu), chi(X))
-    // odd X:  (n1, o1) = (psi(X+mu), tA(X)), (n2, o2) = (chi(X+mu), tB(X))
-    cx<double> n1[4][3], o1[4][3], n2[4][3], o2[4][3];
-    if (p == 0) {
-        ld_site<double>(A.tA, S_, kn, n1);
-        ld_site<double>(A.psi, S_, k, o1);
-        ld_site<double>(A.tB, S_, kn, n2);
-        ld_site<double>(A.chi, S_, k, o2);
-    } else {
-        ld_site<double>(A.psi, S_, kn, n1);
-        ld_site<double>(A.tA, S_, k, o1);
-        ld_site<double>(A.chi, S_, kn, n2);
-        ld_site<double>(A.tB, S_, k, o2);
-    }
     cx<double> F[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) F[i][j] = {0.0, 0.0};
     switch (mu) {
-        case 0: link_force<0, S>(F, n1, o1, n2, o2, p == 0, bc); break;
-        case 1: link_force<1, S>(F, n1, o1, n2, o2, p == 0, bc); break;
-        case 2: link_force<2, S>(F, n1, o1, n2, o2, p == 0, bc); break;
-        default: link_force<3, S>(F, n1, o1, n2, o2, p == 0, bc); break;
+        case 0: link_force<0, S>(F, A, p, k, kn, bc); break;
+        case 1: link_force<1, S>(F, A, p, k, kn, bc); break;
+        case 2: link_force<2, S>(F, A, p, k, kn, bc); break;
+        default: link_force<3, S>(F, A, p, k, kn, bc); break;
     }
     const size_t base = (size_t)((p * 4 + mu) * 9) * S_ + k;
     if constexpr (MODE == FORCE_OUT) {
@@ -150,7 +153,8 @@ hipError_t launch_force(const Geom &g, int sign, int momentum, const void *chi, 
     A.kappa = kappa;
     A.coef = coef;
     A.g = g;
-    const dim3 grid((unsigned)((2 * (size_t)g.vh + kBlock - 1) / kBlock), 4);
+    static_assert(kBlock == 4 * 64, "one wave per direction");
+    const dim3 grid((unsigned)((2 * (size_t)g.vh + 63) / 64));
     if (sign > 0) {
         if (momentum)
             hipLaunchKernelGGL((k_force<1, FORCE_MOMENTUM>), grid, dim3(kBlock), 0, s, A);

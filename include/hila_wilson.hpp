@@ -90,6 +90,24 @@ class GaugeField {
     }
     Lattice &lattice() { return lat_; }
 
+    // gauge_field_base::momentum (hmc/gauge_field.h:62), held on the GPU with the lattice:
+    // one 3x3 complex matrix per link, V x 18 doubles per direction.
+    using LinkField = std::array<std::vector<double>, 4>;
+    void upload_momentum(const LinkField &P) {
+        const double *p[4] = {P[0].data(), P[1].data(), P[2].data(), P[3].data()};
+        check(wl_momentum_upload(lat_.ctx(), p), "wl_momentum_upload");
+    }
+    LinkField download_momentum() const {
+        LinkField P;
+        double *p[4];
+        for (int mu = 0; mu < 4; ++mu) {
+            P[mu].resize(18 * lat_.volume());
+            p[mu] = P[mu].data();
+        }
+        check(wl_momentum_download(lat_.ctx(), p), "wl_momentum_download");
+        return P;
+    }
+
   private:
     Lattice &lat_;
     std::vector<std::vector<double>> U_;
@@ -173,6 +191,17 @@ class Dirac_Wilson_evenodd {
 
     void apply(const WilsonField &in, WilsonField &out) { meo(in, out, +1); }
     void dagger(const WilsonField &in, WilsonField &out) { meo(in, out, -1); }
+    // force(chi, psi, force, sign) (wilson.h:294-311): the derivative of Re <chi, M_sign psi>
+    // with respect to the links, V x 18 doubles per direction
+    void force(const WilsonField &chi, const WilsonField &psi, GaugeField::LinkField &force, int sign) {
+        double *p[4];
+        for (int mu = 0; mu < 4; ++mu) {
+            force[mu].resize(18 * gauge_.lattice().volume());
+            p[mu] = force[mu].data();
+        }
+        check(wl_fermion_force(gauge_.lattice().ctx(), chi.part(EVEN), psi.part(EVEN), kappa, sign, p),
+              "wl_fermion_force");
+    }
     GaugeField &gauge() { return gauge_; }
     double shift() const { return 0.0; }  // the operator is M itself
 
@@ -203,6 +232,9 @@ class Hasenbusch_operator {
 
     void apply(const vector_type &in, vector_type &out) { op(in, out, +1); }
     void dagger(const vector_type &in, vector_type &out) { op(in, out, -1); }
+    void force(const vector_type &chi, const vector_type &psi, GaugeField::LinkField &f, int sign) {
+        D.force(chi, psi, f, sign);  // Hasenbusch.h:50-53
+    }
     GaugeField &gauge() { return D.gauge(); }
     double shift() const { return h_parameter; }
 
@@ -273,6 +305,92 @@ class CG {
     double maxiters = 10000;   // CG_DEFAULT_MAXITERS (conjugate_gradient.h:15)
     int inner_prec_ = WL_F64;
     double delta_ = 1e-3;
+    int update_every_ = 100;
+};
+
+// fermion_action<gauge_field, DIRAC_OP> (libraries/hmc/fermion_field.h:20-168): the
+// pseudofermion chi of one HMC action term with its MRE basis of previous solutions.
+template <typename DIRAC_OP>
+class fermion_action {
+  public:
+    GaugeField &gauge;
+    DIRAC_OP &D;
+    WilsonField chi;
+    int MRE_size = 0;
+    std::vector<WilsonField> old_chi_inv;  // newest first
+    wl_cg_stats stats{};
+
+    fermion_action(DIRAC_OP &d, GaugeField &g, int mre_guess_size = 0)
+        : gauge(g), D(d), chi(g.lattice()), MRE_size(mre_guess_size) {
+        chi.zero(ALL);
+    }
+    // CG settings of the solves (HILA: CG_DEFAULT_ACCURACY, FP64)
+    void set_solver(double accuracy, int inner_prec = WL_F64, double delta = 0.1, int update_every = 100) {
+        accuracy_ = accuracy;
+        inner_prec_ = inner_prec;
+        delta_ = delta;
+        update_every_ = update_every;
+    }
+
+    // initial_guess (:62-86): the MRE guess from the saved solutions (the FP32 pre-solve of
+    // HILA's double-precision builds is replaced by the reliable-update solver, set_solver)
+    void initial_guess(const WilsonField &c, WilsonField &psi) {
+        psi.zero(ALL);
+        if (MRE_size > 0 && !old_chi_inv.empty()) MRE_guess(psi, c, D, old_chi_inv);
+    }
+    // save_new_solution (:134-142): newest first, the oldest dropped
+    void save_new_solution(const WilsonField &psi) {
+        if (MRE_size <= 0) return;
+        old_chi_inv.insert(old_chi_inv.begin(), psi);
+        if ((int)old_chi_inv.size() > MRE_size) old_chi_inv.pop_back();
+    }
+    // draw_gaussian_fields (:119-132): psi Gaussian on D.par, chi = D^dag psi
+    void draw_gaussian_fields(uint64_t seed) {
+        WilsonField psi(gauge.lattice());
+        psi.gaussian_random(seed, EVEN);
+        D.dagger(psi, chi);
+    }
+    // action (:88-103): Re <chi, (D^dag D)^-1 chi> on D.par
+    double action() {
+        WilsonField psi(gauge.lattice());
+        initial_guess(chi, psi);
+        solve(psi);
+        const std::vector<double> a = chi.host(), b = psi.host();
+        double s = 0;
+        for (size_t i = 0; i < a.size() / 2; ++i) s += a[i] * b[i];  // even sites come first
+        return s;
+    }
+    // force_step (:145-167): psi = (D^dag D)^-1 chi, momentum += P(U (-eps)(F + F2)), one
+    // GPU call after the initial guess
+    void force_step(double eps) {
+        WilsonField psi(gauge.lattice());
+        initial_guess(chi, psi);
+        const wl_cg_params p = params();
+        const int rc = wl_fermion_force_step(gauge.lattice().ctx(), chi.part(EVEN), psi.part(EVEN), eps, &p, &stats);
+        if (rc != WL_E_NOCONV) check(rc, "wl_fermion_force_step");
+        save_new_solution(psi);
+    }
+
+  private:
+    wl_cg_params params() const {
+        wl_cg_params p{};
+        p.kappa = D.kappa;
+        p.mh = D.shift();
+        p.accuracy = accuracy_;
+        p.maxiters = 10000;
+        p.inner_prec = inner_prec_;
+        p.delta = delta_;
+        p.update_every = update_every_;
+        return p;
+    }
+    void solve(WilsonField &psi) {
+        const wl_cg_params p = params();
+        const int rc = wl_cg_solve_ex(gauge.lattice().ctx(), psi.part(EVEN), chi.part(EVEN), &p, &stats);
+        if (rc != WL_E_NOCONV) check(rc, "wl_cg_solve_ex");
+    }
+    double accuracy_ = 1e-12;
+    int inner_prec_ = WL_F64;
+    double delta_ = 0.1;
     int update_every_ = 100;
 };
 

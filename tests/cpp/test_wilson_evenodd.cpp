@@ -193,12 +193,78 @@ static void run_hasenbusch(Lattice &lattice) {
     CHECK(iters[3] * 4 <= iters[0], "MRE guess from three previous solutions");
 }
 
+// fermion_action::force_step (fermion_field.h:145-167) against the same step composed from
+// its parts: psi from CG, Dpsi = D psi, F = D.force(Dpsi, psi, +1) and
+// momentum += project_antihermitean(U (-2 eps) F) on the host; then a second step, whose MRE
+// guess from the first solution (same chi) leaves CG almost nothing to do.
+static void run_force(Lattice &lattice) {
+    std::printf("Checking fermion_action::force_step\n");
+    GaugeField U(lattice);
+    U.random(2, 0.3, 31);
+    using dirac = Dirac_Wilson_evenodd<SU3d>;
+    dirac D(0.124, U);
+    fermion_action<dirac> fa(D, U, 3);
+    fa.draw_gaussian_fields(32);
+    const int64_t V = lattice.volume();
+    GaugeField::LinkField zero;
+    for (auto &v : zero) v.assign(18 * V, 0.0);
+    U.upload_momentum(zero);
+    const double eps = 0.01;
+    fa.force_step(eps);
+    const int first_iters = fa.stats.iterations;
+    const GaugeField::LinkField P1 = U.download_momentum();
+
+    WilsonField psi(lattice), Dpsi(lattice);
+    psi.zero(ALL);
+    CG<dirac> inverse(D, 1e-12);
+    inverse.apply(fa.chi, psi);
+    D.apply(psi, Dpsi);
+    GaugeField::LinkField F;
+    D.force(Dpsi, psi, F, +1);
+    double err = 0, ref = 0, ah = 0;
+    for (int mu = 0; mu < 4; ++mu) {
+        const std::vector<double> &u = U.direction(mu);
+        for (int64_t x = 0; x < V; ++x) {
+            std::complex<double> m[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    std::complex<double> a = 0;
+                    for (int k = 0; k < 3; ++k)
+                        a += std::complex<double>(u[18 * x + 2 * (3 * i + k)], u[18 * x + 2 * (3 * i + k) + 1]) *
+                             std::complex<double>(F[mu][18 * x + 2 * (3 * k + j)], F[mu][18 * x + 2 * (3 * k + j) + 1]);
+                    m[i][j] = -2.0 * eps * a;
+                }
+            std::complex<double> tr = 0;
+            std::complex<double> p[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) p[i][j] = 0.5 * (m[i][j] - std::conj(m[j][i]));
+            for (int i = 0; i < 3; ++i) tr += p[i][i];
+            for (int i = 0; i < 3; ++i) p[i][i] -= tr / 3.0;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    const std::complex<double> got(P1[mu][18 * x + 2 * (3 * i + j)], P1[mu][18 * x + 2 * (3 * i + j) + 1]);
+                    err += std::norm(got - p[i][j]);
+                    ref += std::norm(p[i][j]);
+                    const std::complex<double> gt(P1[mu][18 * x + 2 * (3 * j + i)], P1[mu][18 * x + 2 * (3 * j + i) + 1]);
+                    ah += std::norm(got + std::conj(gt));
+                }
+        }
+    }
+    CHECK(ref > 0 && err < 1e-24 * ref, "force_step = P(U (-eps)(F + F2)) composed from its parts");
+    CHECK(ah < 1e-26 * ref, "momentum change anti-Hermitian");
+    fa.force_step(eps);
+    std::printf("      CG iterations: first step %d, from the MRE guess %d\n", first_iters, fa.stats.iterations);
+    CHECK(fa.stats.iterations * 4 <= first_iters, "second step starts from the MRE guess");
+    CHECK(fa.action() > 0, "action > 0");
+}
+
 int main() {
     try {
         Lattice lattice({16, 8, 8, 8});  // test_CG.cpp:57, NDIM == 4
         run(lattice, 0, "unit");
         run(lattice, 1, "Haar random");
         run_hasenbusch(lattice);
+        run_force(lattice);
     } catch (const error &e) {
         std::printf("FAIL: exception %s\n", e.what());
         return 2;
