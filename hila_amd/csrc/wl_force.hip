@@ -139,7 +139,25 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs A) {
     }
 }
 
+// y -= x on every site of an even FP64 field of one rhs
+__global__ __launch_bounds__(kBlock) void k_sub(cx<double> *__restrict__ y, const cx<double> *__restrict__ x, int vh,
+                                                int S) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= vh) return;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+        const size_t e = (size_t)c * S + k;
+        y[e] = y[e] - x[e];
+    }
+}
+
 }  // namespace
+
+hipError_t launch_sub(const Geom &g, void *y, const void *x, hipStream_t s) {
+    hipLaunchKernelGGL(k_sub, dim3((g.vh + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (cx<double> *)y,
+                       (const cx<double> *)x, g.vh, g.stride);
+    return hipGetLastError();
+}
 
 hipError_t launch_force(const Geom &g, int sign, int momentum, const void *chi, const void *psi, const void *tA,
                         const void *tB, const void *gauge, void *out, double kappa, double coef, hipStream_t s) {
@@ -264,6 +282,53 @@ extern "C" int wl_momentum_download(wl_ctx *c, double *const P[4]) {
     DeviceGuard dg(c->device);
     if (int rc = ensure_links(c, &c->mom64, "momentum")) return rc;
     return links_download(c, c->mom64, P);
+}
+
+extern "C" int wl_hasenbusch2_force_step(wl_ctx *c, const wl_field *chi, wl_field *psi, double eps,
+                                         const wl_cg_params *p, wl_cg_stats *stats) {
+    if (!c || !p) return fail(WL_E_ARG, "null argument");
+    if (int rc = check_vec(c, chi, "chi")) return rc;
+    if (int rc = check_vec(c, psi, "psi")) return rc;
+    if (chi == psi) return fail(WL_E_ARG, "chi and psi must be different fields");
+    if (!std::isfinite(eps) || !std::isfinite(p->mh)) return fail(WL_E_ARG, "eps and mh must be finite");
+    // Dhchi = D_h^dag chi (fermion_field.h:331), kept in a work field of the context, wrapped as
+    // a one-rhs field for the solver
+    wl_field *src;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        DeviceGuard dg(c->device);
+        int rc;
+        if ((rc = ensure_gauge(c, WL_F64)) || (rc = work_field(c, W_SRC, WL_F64, WL_EVEN, 1, &src)) ||
+            (rc = ensure_tmp(c, WL_F64, &c->tmp_odd64, WL_ODD)))
+            return rc;
+        OpShift os(c, p->mh);
+        if ((rc = meo_one(c, WL_F64, src->data, chi->data, c->tmp_odd64->data, p->kappa, -1, EPI_AXPY, nullptr,
+                          FIN_NONE)))
+            return rc;
+    }
+    // psi = (D^dag D)^-1 Dhchi with the unshifted operator (CG<DIRAC_OP> inverse(D), :327, :333-334)
+    wl_cg_params q = *p;
+    q.mh = 0.0;
+    wl_field view = *src;
+    view.nrhs = 1;
+    const int solved = wl_cg_solve_ex(c, psi, &view, &q, stats);
+    if (solved != WL_OK && solved != WL_E_NOCONV) return solved;
+    std::lock_guard<std::mutex> lk(c->mu);
+    DeviceGuard dg(c->device);
+    int rc;
+    wl_field *mpsi;
+    if ((rc = work_field(c, W_DP64, WL_F64, WL_EVEN, 1, &mpsi)) || (rc = ensure_links(c, &c->mom64, "momentum")))
+        return rc;
+    // Mpsi = D psi - chi (:337-339)
+    if ((rc = meo_one(c, WL_F64, mpsi->data, psi->data, c->tmp_odd64->data, p->kappa, +1, EPI_AXPY, nullptr,
+                      FIN_NONE)))
+        return rc;
+    HIPCHK(launch_sub(c->g, mpsi->data, chi->data, c->stream));
+    // force = -eps (D.force(Mpsi, psi, +1) + D.force(psi, Mpsi, -1)) = -2 eps D.force(Mpsi, psi, +1)
+    // (:341-344), add_momentum fused
+    if ((rc = force_launch(c, mpsi->data, psi->data, p->kappa, +1, 1, c->mom64, -2.0 * eps))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return solved;
 }
 
 extern "C" int wl_fermion_force_step(wl_ctx *c, const wl_field *chi, wl_field *psi, double eps,

@@ -73,6 +73,8 @@ hipError_t launch_links_to_host(const Geom &g, const void *planes, int mu, doubl
 // or out += project_antihermitean(U coef F) (momentum != 0)
 hipError_t launch_force(const Geom &g, int sign, int momentum, const void *chi, const void *psi, const void *tA,
                         const void *tB, const void *gauge, void *out, double kappa, double coef, hipStream_t s);
+// y -= x for even FP64 fields of one rhs (wl_force.hip)
+hipError_t launch_sub(const Geom &g, void *y, const void *x, hipStream_t s);
 // sum over sites of ||f_j||^2 for rhs j of an N-rhs field (FP64 accumulate), then `fin`
 hipError_t launch_norm2(int prec, const Geom &g, const void *f, int N, int j, const RedArgs &ra,
                         int fin, const int *done, hipStream_t s);

@@ -41,6 +41,7 @@ EXPORTED_SYMBOLS = [
     "wl_gauge_download", "wl_config_header", "wl_config_read", "wl_config_write",
     "wl_hasenbusch_apply", "wl_cg_solve_ex", "wl_mre_guess", "wl_field_copy_rhs",
     "wl_fermion_force", "wl_momentum_upload", "wl_momentum_download", "wl_fermion_force_step",
+    "wl_hasenbusch2_force_step",
 ]
 
 
@@ -159,6 +160,7 @@ def lib():
         L.wl_momentum_upload.argtypes = [vp, C.POINTER(vp)]
         L.wl_momentum_download.argtypes = [vp, C.POINTER(vp)]
         L.wl_fermion_force_step.argtypes = [vp, vp, vp, d, C.POINTER(CGParams), C.POINTER(CGStats)]
+        L.wl_hasenbusch2_force_step.argtypes = [vp, vp, vp, d, C.POINTER(CGParams), C.POINTER(CGStats)]
         for name in EXPORTED_SYMBOLS:
             if name not in ("wl_version", "wl_last_error", "wl_volume"):
                 getattr(L, name).restype = C.c_int
@@ -470,6 +472,20 @@ def force_step(D, chi: Field, psi: Field, eps, accuracy=1e-12, maxiters=10000, p
     st = CGStats()
     check(lib().wl_fermion_force_step(D.lat.handle, chi.handle, psi.handle, float(eps), C.byref(p),
                                       C.byref(st)), allow=(WL_E_NOCONV,))
+    return st.as_dict()
+
+
+def hasenbusch2_force_step(H, chi: Field, psi: Field, eps, accuracy=1e-12, maxiters=10000,
+                           precision="f64", delta=0.1, update_every=100):
+    """Hasenbusch_action_2::force_step(eps) (fermion_field.h:319-346) for the Hasenbusch
+    operator H = D + mh: psi = (D^dag D)^-1 H^dag chi, Mpsi = D psi - chi, momentum +=
+    P(U force).  Returns the CG stats dict."""
+    p = CGParams(kappa=H.kappa, mh=H.mh, accuracy=float(accuracy), maxiters=int(maxiters),
+                 inner_prec={"f64": F64, "f32": F32, "f16": F16}[precision],
+                 delta=float(delta), update_every=int(update_every))
+    st = CGStats()
+    check(lib().wl_hasenbusch2_force_step(H.lat.handle, chi.handle, psi.handle, float(eps),
+                                          C.byref(p), C.byref(st)), allow=(WL_E_NOCONV,))
     return st.as_dict()
 
 

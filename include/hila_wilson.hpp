@@ -394,6 +394,92 @@ class fermion_action {
     int update_every_ = 100;
 };
 
+// Hasenbusch_action_1 (libraries/hmc/fermion_field.h:168-196): the fermion action of
+// D_h = D + mh.
+template <typename DIRAC_OP>
+class Hasenbusch_action_1 {
+  public:
+    Hasenbusch_operator<DIRAC_OP> D_h;
+    fermion_action<Hasenbusch_operator<DIRAC_OP>> base_action;
+
+    Hasenbusch_action_1(DIRAC_OP &d, GaugeField &g, double mh, int mre_guess_size = 0)
+        : D_h(d, mh), base_action(D_h, g, mre_guess_size) {}
+    double action() { return base_action.action(); }
+    void draw_gaussian_fields(uint64_t seed) { base_action.draw_gaussian_fields(seed); }
+    void force_step(double eps) { base_action.force_step(eps); }
+};
+
+// Hasenbusch_action_2 (libraries/hmc/fermion_field.h:198-368): the second term of the split,
+// action <chi, D_h (D^dag D)^-1 D_h^dag chi>.
+template <typename DIRAC_OP>
+class Hasenbusch_action_2 {
+  public:
+    GaugeField &gauge;
+    DIRAC_OP D;
+    Hasenbusch_operator<DIRAC_OP> D_h;
+    double mh;
+    WilsonField chi;
+    int MRE_size = 0;
+    std::vector<WilsonField> old_chi_inv;  // newest first
+    wl_cg_stats stats{};
+
+    Hasenbusch_action_2(DIRAC_OP &d, GaugeField &g, double _mh, int mre_guess_size = 0)
+        : gauge(g), D(d), D_h(d, _mh), mh(_mh), chi(g.lattice()), MRE_size(mre_guess_size) {
+        chi.zero(ALL);
+    }
+
+    // action (:255-272): |D (D^dag D)^-1 D_h^dag chi|^2 on D.par
+    double action() {
+        WilsonField psi(gauge.lattice()), v(gauge.lattice());
+        D_h.dagger(chi, psi);
+        v.zero(ALL);
+        CG<DIRAC_OP> inverse(D, accuracy_);
+        inverse.apply(psi, v);
+        D.apply(v, psi);
+        stats = inverse.stats;
+        return psi.squarenorm(EVEN);
+    }
+    // draw_gaussian_fields (:274-291): chi = D_h (D_h^dag D_h)^-1 D^dag psi, psi Gaussian on D.par
+    void draw_gaussian_fields(uint64_t seed) {
+        WilsonField psi(gauge.lattice()), v(gauge.lattice());
+        psi.gaussian_random(seed, EVEN);
+        D.dagger(psi, v);
+        psi.zero(ALL);
+        CG<Hasenbusch_operator<DIRAC_OP>> inverse_h(D_h, accuracy_);
+        inverse_h.apply(v, psi);
+        D_h.apply(psi, chi);
+    }
+    // initial_guess (:293-317): the MRE guess for (D^dag D) psi = c
+    void initial_guess(const WilsonField &c, WilsonField &psi) {
+        psi.zero(ALL);
+        if (MRE_size > 0 && !old_chi_inv.empty()) MRE_guess(psi, c, D, old_chi_inv);
+    }
+    void save_new_solution(const WilsonField &psi) {
+        if (MRE_size <= 0) return;
+        old_chi_inv.insert(old_chi_inv.begin(), psi);
+        if ((int)old_chi_inv.size() > MRE_size) old_chi_inv.pop_back();
+    }
+    // force_step (:319-346), one GPU call after the initial guess
+    void force_step(double eps) {
+        WilsonField Dhchi(gauge.lattice()), psi(gauge.lattice());
+        D_h.dagger(chi, Dhchi);
+        initial_guess(Dhchi, psi);
+        wl_cg_params p{};
+        p.kappa = D.kappa;
+        p.mh = mh;
+        p.accuracy = accuracy_;
+        p.maxiters = 10000;
+        p.inner_prec = WL_F64;
+        const int rc = wl_hasenbusch2_force_step(gauge.lattice().ctx(), chi.part(EVEN), psi.part(EVEN), eps, &p, &stats);
+        if (rc != WL_E_NOCONV) check(rc, "wl_hasenbusch2_force_step");
+        save_new_solution(psi);
+    }
+    void set_accuracy(double a) { accuracy_ = a; }
+
+  private:
+    double accuracy_ = 1e-12;
+};
+
 // Request server (wl_server_*): independent solves admitted by HBM budget and run
 // concurrently on one GPU.  Host buffers of a request must outlive its wait().
 class Server {

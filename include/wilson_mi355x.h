@@ -258,6 +258,17 @@ int wl_momentum_download(wl_ctx *ctx, double *const P[4]);
 int wl_fermion_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, double eps,
                           const wl_cg_params *params, wl_cg_stats *stats);
 
+/* Hasenbusch_action_2::force_step(eps) (libraries/hmc/fermion_field.h:319-346), the second
+ * term of the Hasenbusch split, action <chi, D_h (D^dag D)^-1 D_h^dag chi> with
+ * D_h = D + params->mh:
+ *   psi = CG solution of (D^dag D) psi = D_h^dag chi (unshifted D; psi holds the initial
+ *         guess on entry and the solution on exit);
+ *   Mpsi = D psi - chi;
+ *   momentum += project_antihermitean(U (-eps)(D.force(Mpsi, psi, +1) + D.force(psi, Mpsi, -1))).
+ * Returns as wl_fermion_force_step. */
+int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, double eps,
+                              const wl_cg_params *params, wl_cg_stats *stats);
+
 /* ---- kernel options ------------------------------------------------------------------- */
 
 /* Tuning knobs of the hopping kernel (no counterpart in HILA, whose GPU loops are generated

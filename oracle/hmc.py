@@ -51,3 +51,14 @@ def force_step(orc, U, kappa, mh, chi, psi, mom, eps):
     F1 = orc.force(U, kappa, Mpsi, psi, 1)
     F2 = orc.force(U, kappa, psi, Mpsi, -1)
     return add_momentum(U, -eps * (F1 + F2), mom)
+
+
+def hasenbusch2_force_step(orc, U, kappa, mh, chi, psi, mom, eps):
+    """The momentum after Hasenbusch_action_2::force_step (fermion_field.h:319-346), for psi
+    already solved from (D^dag D) psi = D_h^dag chi: Mpsi = D psi - chi on D.par, then the
+    same force and add_momentum as force_step."""
+    Mpsi = orc.meo(U, kappa, psi, 1)
+    Mpsi[: orc.Vh] -= chi[: orc.Vh]
+    F1 = orc.force(U, kappa, Mpsi, psi, 1)
+    F2 = orc.force(U, kappa, psi, Mpsi, -1)
+    return add_momentum(U, -eps * (F1 + F2), mom)

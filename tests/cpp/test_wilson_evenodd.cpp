@@ -258,6 +258,43 @@ static void run_force(Lattice &lattice) {
     CHECK(fa.action() > 0, "action > 0");
 }
 
+// The two Hasenbusch terms (fermion_field.h:168-368): after draw_gaussian_fields, the second
+// term's action is the squared norm of its Gaussian draw (chi = D_h (D_h^dag D_h)^-1 D^dag psi
+// gives D (D^dag D)^-1 D_h^dag chi = psi); both force steps move the momentum within the algebra.
+static void run_hasenbusch_actions(Lattice &lattice) {
+    std::printf("Checking Hasenbusch_action_1 / Hasenbusch_action_2\n");
+    GaugeField U(lattice);
+    U.random(2, 0.4, 41);
+    using dirac = Dirac_Wilson_evenodd<SU3d>;
+    dirac D(0.124, U);
+    Hasenbusch_action_1<dirac> fa1(D, U, 0.2, 2);
+    Hasenbusch_action_2<dirac> fa2(D, U, 0.2, 2);
+    fa1.draw_gaussian_fields(42);
+    fa2.draw_gaussian_fields(43);
+    WilsonField drawn(lattice);
+    drawn.gaussian_random(43, EVEN);
+    const double a2 = fa2.action(), n2 = drawn.squarenorm(EVEN);
+    CHECK(std::abs(a2 - n2) < 1e-8 * n2, "Hasenbusch_action_2: action = |drawn psi|^2");
+    const int64_t V = lattice.volume();
+    GaugeField::LinkField zero;
+    for (auto &v : zero) v.assign(18 * V, 0.0);
+    U.upload_momentum(zero);
+    fa1.force_step(0.01);
+    fa2.force_step(0.01);
+    const GaugeField::LinkField P = U.download_momentum();
+    double norm = 0, ah = 0;
+    for (int mu = 0; mu < 4; ++mu)
+        for (int64_t x = 0; x < V; ++x)
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    const std::complex<double> a(P[mu][18 * x + 2 * (3 * i + j)], P[mu][18 * x + 2 * (3 * i + j) + 1]);
+                    const std::complex<double> b(P[mu][18 * x + 2 * (3 * j + i)], P[mu][18 * x + 2 * (3 * j + i) + 1]);
+                    norm += std::norm(a);
+                    ah += std::norm(a + std::conj(b));
+                }
+    CHECK(norm > 0 && ah < 1e-26 * norm, "Hasenbusch force steps: momentum change anti-Hermitian");
+}
+
 int main() {
     try {
         Lattice lattice({16, 8, 8, 8});  // test_CG.cpp:57, NDIM == 4
@@ -265,6 +302,7 @@ int main() {
         run(lattice, 1, "Haar random");
         run_hasenbusch(lattice);
         run_force(lattice);
+        run_hasenbusch_actions(lattice);
     } catch (const error &e) {
         std::printf("FAIL: exception %s\n", e.what());
         return 2;

@@ -76,3 +76,24 @@ def test_force_step_matches_oracle(mh, precision):
     dP = hmc.as_matrices(P1 - P0)
     assert np.abs(dP + np.conj(np.swapaxes(dP, -1, -2))).max() < 1e-13 * np.abs(dP).max()
     assert np.abs(np.trace(dP, axis1=-2, axis2=-1)).max() < 1e-13 * np.abs(dP).max()
+
+
+def test_hasenbusch2_force_step_matches_oracle():
+    dims, kappa, mh, eps = (8, 8, 8, 8), 0.124, 0.2, 0.05
+    _say("Hasenbusch_action_2 force step")
+    U = W.gen_gauge(dims, "smooth", 0.3, 820)
+    chi = W.gen_spinor(dims, 821, EVEN)
+    with W.Lattice(dims) as lat:
+        lat.upload_gauge(U)
+        H = W.Hasenbusch_operator(W.Dirac_Wilson_evenodd(kappa, lat), mh)
+        psi = W.Field(lat)
+        st = W.hasenbusch2_force_step(H, W.Field(lat).upload(chi), psi, eps, accuracy=1e-10)
+        P1 = lat.download_momentum()
+        psi_h = psi.to_full()
+    assert st["converged"] == 1
+    orc = Oracle(dims)
+    b = orc.meo(U, kappa, chi, -1, mh=mh)              # D_h^dag chi
+    r = b - orc.meo(U, kappa, orc.meo(U, kappa, psi_h, 1), -1)
+    assert np.linalg.norm(r) <= 1.5e-10 * np.linalg.norm(b)
+    want = hmc.hasenbusch2_force_step(orc, U, kappa, mh, chi, psi_h, np.zeros_like(P1), eps)
+    assert rel(P1, want) < 1e-12

@@ -12,7 +12,8 @@ HILA holds no fixture for the force; what fixes it is that it is a derivative:
   * D.force(a, b, s) = D.force(b, a, -s) (the two calls of force_step form the same products);
   * the force step's momentum change is the gradient of the action S = <chi, (D^dag D)^-1 chi>:
     for U_mu(x) -> exp(e T) U_mu(x), T traceless anti-Hermitian, dS/de = Re tr(T dP) / eps,
-    which also fixes project_antihermitean (undefined in the reference snapshot).
+    which also fixes project_antihermitean (undefined in the reference snapshot); the same for
+    the second Hasenbusch term, S2 = <chi, D_h (D^dag D)^-1 D_h^dag chi>, D_h = D + mh.
 """
 import numpy as np
 import pytest
@@ -127,6 +128,39 @@ def test_force_step_is_the_gradient_of_the_action(mh):
         A = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
         T = hmc.project_antihermitean(A)
         Um = hmc.as_matrices(U)
+
+        def moved(e):
+            U2 = U.copy()
+            U2[mu, x] = hmc.as_real(sla.expm(e * T) @ Um[mu, x])
+            return U2
+
+        dS = (action(moved(h)) - action(moved(-h))) / (2 * h)
+        want = np.trace(T @ dP[mu, x]).real / eps
+        assert abs(dS - want) < 1e-6 * max(1.0, abs(want)), (mu, x, dS, want)
+
+
+def test_hasenbusch2_force_step_is_the_gradient_of_its_action():
+    dims, mh = (4, 4, 4, 4), 0.2
+    U = W.gen_gauge(dims, "smooth", 0.5, 730)
+    orc = Oracle(dims)
+    chi = W.gen_spinor(dims, 731, EVEN)
+
+    def solve(UU):
+        Dhchi = orc.meo(UU, KAPPA, chi, -1, mh=mh)
+        x, _, _ = orc.cg(UU, KAPPA, Dhchi, accuracy=1e-14)
+        return x
+
+    def action(UU):   # Hasenbusch_action_2::action (:255-272): |D (D^dag D)^-1 D_h^dag chi|^2
+        v = orc.meo(UU, KAPPA, solve(UU), 1)
+        return float(np.sum(v[: orc.Vh] ** 2))
+
+    eps = 0.5
+    dP = hmc.as_matrices(hmc.hasenbusch2_force_step(orc, U, KAPPA, mh, chi, solve(U), np.zeros_like(U), eps))
+    rng = np.random.default_rng(11)
+    h = 1e-5
+    Um = hmc.as_matrices(U)
+    for mu, x in [(0, 7), (2, orc.Vh + 9), (3, orc.V - 1)]:
+        T = hmc.project_antihermitean(rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3)))
 
         def moved(e):
             U2 = U.copy()
