@@ -59,19 +59,47 @@ template <typename R>
 __device__ __forceinline__ cx<R> cmulc(cx<R> a, cx<R> b) {
     return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
 }
-// acc + a * b
+// a * b + c, fused
+__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// acc + a * b.  In FP32 as four fused multiply-adds: written as acc + (a.x b.x - a.y b.y) the
+// compiler forms the bracket first and adds it, two more instructions per term, which matters
+// in the FP32/FP16 kernels, where the VALU is busy.  In FP64 (memory-bound) the bracket form is
+// kept: the chained form's longer dependences raise the FP64 kernels' register use.
 template <typename R>
 __device__ __forceinline__ cx<R> cfma(cx<R> acc, cx<R> a, cx<R> b) {
-    acc.x += a.x * b.x - a.y * b.y;
-    acc.y += a.x * b.y + a.y * b.x;
+    if constexpr (sizeof(R) == 4) {
+        acc.x = fmar(a.x, b.x, acc.x);
+        acc.x = fmar(-a.y, b.y, acc.x);
+        acc.y = fmar(a.x, b.y, acc.y);
+        acc.y = fmar(a.y, b.x, acc.y);
+    } else {
+        acc.x += a.x * b.x - a.y * b.y;
+        acc.y += a.x * b.y + a.y * b.x;
+    }
     return acc;
 }
-// acc + conj(a) * b
+// acc + conj(a) * b, likewise
 template <typename R>
 __device__ __forceinline__ cx<R> cfmac(cx<R> acc, cx<R> a, cx<R> b) {
-    acc.x += a.x * b.x + a.y * b.y;
-    acc.y += a.x * b.y - a.y * b.x;
+    if constexpr (sizeof(R) == 4) {
+        acc.x = fmar(a.x, b.x, acc.x);
+        acc.x = fmar(a.y, b.y, acc.x);
+        acc.y = fmar(a.x, b.y, acc.y);
+        acc.y = fmar(-a.y, b.x, acc.y);
+    } else {
+        acc.x += a.x * b.x + a.y * b.y;
+        acc.y += a.x * b.y - a.y * b.x;
+    }
     return acc;
+}
+// acc + f z (FP32: fused; FP64: product then sum, as cfma)
+template <typename R>
+__device__ __forceinline__ cx<R> caxpy(cx<R> acc, R f, cx<R> z) {
+    if constexpr (sizeof(R) == 4)
+        return {fmar(f, z.x, acc.x), fmar(f, z.y, acc.y)};
+    else
+        return acc + f * z;
 }
 // s * i * a  with s = +-1 folded at compile time
 template <int S, typename R>
@@ -159,6 +187,31 @@ __device__ __forceinline__ void expand_add(cx<R> (&acc)[4][3], const cx<R> (&h)[
         } else {
             acc[2][c] = acc[2][c] + sr<S>(h[0][c]);
             acc[3][c] = acc[3][c] + sr<S>(h[1][c]);
+        }
+    }
+}
+
+// acc += f expand(h, MU, S): expand_add of the half spinor scaled by f (a site scale or the
+// antiperiodic sign), with the scaling folded into the accumulation as fused multiply-adds.
+template <int MU, int S, typename R>
+__device__ __forceinline__ void expand_add_scaled(cx<R> (&acc)[4][3], const cx<R> (&h)[2][3], R f) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const cx<R> h0 = h[0][c], h1 = h[1][c];
+        acc[0][c] = caxpy(acc[0][c], f, h0);
+        acc[1][c] = caxpy(acc[1][c], f, h1);
+        if constexpr (MU == 0) {
+            acc[2][c] = caxpy(acc[2][c], -f, si<S>(h1));
+            acc[3][c] = caxpy(acc[3][c], -f, si<S>(h0));
+        } else if constexpr (MU == 1) {
+            acc[2][c] = caxpy(acc[2][c], f, sr<S>(h1));
+            acc[3][c] = caxpy(acc[3][c], -f, sr<S>(h0));
+        } else if constexpr (MU == 2) {
+            acc[2][c] = caxpy(acc[2][c], -f, si<S>(h0));
+            acc[3][c] = caxpy(acc[3][c], f, si<S>(h1));
+        } else {
+            acc[2][c] = caxpy(acc[2][c], f, sr<S>(h0));
+            acc[3][c] = caxpy(acc[3][c], f, sr<S>(h1));
         }
     }
 }

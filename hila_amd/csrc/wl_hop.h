@@ -50,15 +50,16 @@ __device__ __forceinline__ void hop_compute(cx<R> (&acc)[4][3], const HopBuf<R> 
         su3_adj_mul(b.u, h, v);
     }
     if constexpr (MU == 3) {
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) v[q][c] = sg * v[q][c];
+        if constexpr (FWD)
+            expand_add_scaled<MU, SIGN>(acc, v, sg);
+        else
+            expand_add_scaled<MU, -SIGN>(acc, v, sg);
+    } else {
+        if constexpr (FWD)
+            expand_add<MU, SIGN>(acc, v);
+        else
+            expand_add<MU, -SIGN>(acc, v);
     }
-    if constexpr (FWD)
-        expand_add<MU, SIGN>(acc, v);
-    else
-        expand_add<MU, -SIGN>(acc, v);
 }
 
 // Hop H = 2 mu + (0 forward, 1 backward): issue hop H+1's loads, then compute hop H.

@@ -95,15 +95,16 @@ __device__ __forceinline__ void rb_hop(cx<typename St<T>::R> (&acc)[NR][4][3], c
                 su3_adj_mul(u, h, v);
             }
             if constexpr (MU == 3 || St<T>::scaled) {
-#pragma unroll
-                for (int q = 0; q < 2; ++q)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) v[q][c] = sc[r] * v[q][c];
+                if constexpr (FWD)
+                    expand_add_scaled<MU, SIGN>(acc[r], v, sc[r]);
+                else
+                    expand_add_scaled<MU, -SIGN>(acc[r], v, sc[r]);
+            } else {
+                if constexpr (FWD)
+                    expand_add<MU, SIGN>(acc[r], v);
+                else
+                    expand_add<MU, -SIGN>(acc[r], v);
             }
-            if constexpr (FWD)
-                expand_add<MU, SIGN>(acc[r], v);
-            else
-                expand_add<MU, -SIGN>(acc[r], v);
         }
 }
 
@@ -160,15 +161,16 @@ __device__ __forceinline__ void rb_compute_u(cx<typename St<T>::R> (&acc)[NR][4]
             if constexpr (MU == 3 || St<T>::scaled) {
                 R f = MU == 3 ? sg : R(1);
                 if constexpr (St<T>::scaled) f *= b.sc[r];
-#pragma unroll
-                for (int q = 0; q < 2; ++q)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) v[q][c] = f * v[q][c];
+                if constexpr (FWD)
+                    expand_add_scaled<MU, SIGN>(acc[r], v, f);
+                else
+                    expand_add_scaled<MU, -SIGN>(acc[r], v, f);
+            } else {
+                if constexpr (FWD)
+                    expand_add<MU, SIGN>(acc[r], v);
+                else
+                    expand_add<MU, -SIGN>(acc[r], v);
             }
-            if constexpr (FWD)
-                expand_add<MU, SIGN>(acc[r], v);
-            else
-                expand_add<MU, -SIGN>(acc[r], v);
         }
 }
 
