@@ -56,6 +56,20 @@ struct RbArgs {
 // does not displace the neighbour spinors from the caches.  For the scaled format the site
 // scale of each neighbour is applied to the product U h (6 complex numbers) instead of to the
 // 12 components of the spinor: the hop is linear.
+// The projected half spinor of a stored neighbour spinor: FP16 storage straight from the halves
+// (project_h2), the other formats widened first.
+template <int MU, int S, typename T>
+__device__ __forceinline__ void rb_project(const typename St<T>::E (&wr)[12], cx<typename St<T>::R> (&h)[2][3]) {
+    if constexpr (St<T>::scaled) {
+        project_h2<MU, S>(wr, h);
+    } else {
+        cx<typename St<T>::R> w[4][3];
+#pragma unroll
+        for (int c = 0; c < 12; ++c) w[c / 3][c % 3] = widen(wr[c]);
+        project<MU, S>(w, h);
+    }
+}
+
 template <int MU, int SIGN, bool FWD, int NR, bool NTL, typename T>
 __device__ __forceinline__ void rb_hop(cx<typename St<T>::R> (&acc)[NR][4][3], const bool (&act)[NR],
                                        const typename St<T>::E *__restrict__ in, size_t rs,
@@ -69,15 +83,12 @@ __device__ __forceinline__ void rb_hop(cx<typename St<T>::R> (&acc)[NR][4][3], c
 #pragma unroll
         for (int e = 0; e < 9; ++e) u[e / 3][e % 3] = widen(ur[e]);
     }
-    cx<R> w[NR][4][3];
+    typename St<T>::E wr[NR][12];
     R sc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r)
         if (act[r]) {
-            typename St<T>::E wr[12];
-            ld_spinor_raw<false, T>(wr, in + r * rs, S, kn);
-#pragma unroll
-            for (int c = 0; c < 12; ++c) w[r][c / 3][c % 3] = widen(wr[c]);
+            ld_spinor_raw<false, T>(wr[r], in + r * rs, S, kn);
             if constexpr (St<T>::scaled)
                 sc[r] = sg * scale_plane(in + r * rs, S)[kn];
             else
@@ -87,13 +98,11 @@ __device__ __forceinline__ void rb_hop(cx<typename St<T>::R> (&acc)[NR][4][3], c
     for (int r = 0; r < NR; ++r)
         if (act[r]) {
             cx<R> h[2][3], v[2][3];
-            if constexpr (FWD) {
-                project<MU, SIGN>(w[r], h);
+            rb_project<MU, FWD ? SIGN : -SIGN, T>(wr[r], h);
+            if constexpr (FWD)
                 su3_mul(u, h, v);
-            } else {
-                project<MU, -SIGN>(w[r], h);
+            else
                 su3_adj_mul(u, h, v);
-            }
             if constexpr (MU == 3 || St<T>::scaled) {
                 if constexpr (FWD)
                     expand_add_scaled<MU, SIGN>(acc[r], v, sc[r]);
@@ -148,16 +157,12 @@ __device__ __forceinline__ void rb_compute_u(cx<typename St<T>::R> (&acc)[NR][4]
 #pragma unroll
     for (int r = 0; r < NR; ++r)
         if (act[r]) {
-            cx<R> w[4][3], h[2][3], v[2][3];
-#pragma unroll
-            for (int c = 0; c < 12; ++c) w[c / 3][c % 3] = widen(b.w[r][c]);
-            if constexpr (FWD) {
-                project<MU, SIGN>(w, h);
+            cx<R> h[2][3], v[2][3];
+            rb_project<MU, FWD ? SIGN : -SIGN, T>(b.w[r], h);
+            if constexpr (FWD)
                 su3_mul(u, h, v);
-            } else {
-                project<MU, -SIGN>(w, h);
+            else
                 su3_adj_mul(u, h, v);
-            }
             if constexpr (MU == 3 || St<T>::scaled) {
                 R f = MU == 3 ? sg : R(1);
                 if constexpr (St<T>::scaled) f *= b.sc[r];

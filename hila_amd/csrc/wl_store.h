@@ -230,6 +230,58 @@ __device__ __forceinline__ void st_site(typename St<T>::E *base, int S, int k,
     }
 }
 
+// ------------------------------------------- spin projection straight from FP16 storage ----
+// half(a, SA) + (NEG ? -1 : 1) half(b, SB) in FP32, where half(v, 0 / 1) is the low (real) /
+// high (imaginary) FP16 half of a stored complex number: one v_fma_mix_f32 (a * 1.0 + b with
+// both FP16 operands widened inside the instruction, one rounding) instead of two conversions
+// and an addition.  The compiler does not form it from (float)a + (float)b by itself.  Against
+// the convert-then-add form the hop's outputs were measured to differ on under 2% of the
+// components, by one FP32 rounding (and so at most one FP16 step where they are stored).
+template <int SA, int SB, bool NEG>
+__device__ __forceinline__ float mix_add(h2 a, h2 b) {
+    const unsigned ua = __builtin_bit_cast(unsigned, a), ub = __builtin_bit_cast(unsigned, b);
+    float r;
+#define WL_MIX(SEL, NEGB)                                                                       \
+    asm("v_fma_mix_f32 %0, %1, 1.0, " NEGB "%2 op_sel:" SEL " op_sel_hi:[1,0,1]" : "=v"(r)      \
+        : "v"(ua), "v"(ub))
+    if constexpr (SA == 0 && SB == 0) {
+        if constexpr (NEG) WL_MIX("[0,0,0]", "-"); else WL_MIX("[0,0,0]", "");
+    } else if constexpr (SA == 0 && SB == 1) {
+        if constexpr (NEG) WL_MIX("[0,0,1]", "-"); else WL_MIX("[0,0,1]", "");
+    } else if constexpr (SA == 1 && SB == 0) {
+        if constexpr (NEG) WL_MIX("[1,0,0]", "-"); else WL_MIX("[1,0,0]", "");
+    } else {
+        if constexpr (NEG) WL_MIX("[1,0,1]", "-"); else WL_MIX("[1,0,1]", "");
+    }
+#undef WL_MIX
+    return r;
+}
+
+// a + c b for stored complex numbers, c = +1 (C == 1), -1 (-1), +i (2) or -i (-2)
+template <int C>
+__device__ __forceinline__ cx<float> mix_cadd(h2 a, h2 b) {
+    if constexpr (C == 1) return {mix_add<0, 0, false>(a, b), mix_add<1, 1, false>(a, b)};
+    else if constexpr (C == -1) return {mix_add<0, 0, true>(a, b), mix_add<1, 1, true>(a, b)};
+    else if constexpr (C == 2) return {mix_add<0, 1, true>(a, b), mix_add<1, 0, false>(a, b)};
+    else return {mix_add<0, 1, false>(a, b), mix_add<1, 0, true>(a, b)};
+}
+
+// project<MU, S> (wl_device.h) of a spinor held in FP16 storage (12 stored complex numbers,
+// component c = spin * 3 + colour), without widening it first.
+template <int MU, int S>
+__device__ __forceinline__ void project_h2(const h2 (&w)[12], cx<float> (&h)[2][3]) {
+    // h0 = w0 + c0 w[b0], h1 = w1 + c1 w[b1]: the coefficients of wl_device.h's project
+    constexpr int b0 = (MU == 0 || MU == 1) ? 3 : 2;
+    constexpr int b1 = (MU == 0 || MU == 1) ? 2 : 3;
+    constexpr int c0 = MU == 0 ? 2 * S : MU == 1 ? -S : MU == 2 ? 2 * S : S;
+    constexpr int c1 = MU == 0 ? 2 * S : MU == 1 ? S : MU == 2 ? -2 * S : S;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        h[0][c] = mix_cadd<c0>(w[c], w[b0 * 3 + c]);
+        h[1][c] = mix_cadd<c1>(w[3 + c], w[b1 * 3 + c]);
+    }
+}
+
 // Host side: precision code -> planes per rhs and bytes per stored element.
 inline constexpr int planes_of(int prec) { return prec == 2 ? 13 : 12; }
 inline constexpr int elem_bytes(int prec) { return prec == 0 ? 16 : (prec == 1 ? 8 : 4); }
