@@ -240,6 +240,22 @@ int rhs_per_launch(const wl_ctx *c, int nr, int N) {
     return rhs_per_launch_of(c->rhs_launch > 0 ? c->rhs_launch : N, nr, N);
 }
 
+// Geometry of one multi-rhs hopping launch of nb rhs.  FP64 launches of four or more rhs, or
+// of two or more on lattices of 2^17 sites per parity or more, hold every workgroup to 32 KB
+// of LDS per 64 lanes (at most five 64-lane workgroups, 320 sites, per CU; workgroups of 64
+// lanes unless the caller chose a size): at full occupancy the neighbour spinors of the sites
+// in flight overflow the 4 MB L2 of an XCD and are fetched from HBM about three times per
+// hop.  Applies only while the caller has set no LDS reserve and the links are not staged in
+// LDS.
+Geom mr_geom(const wl_ctx *c, int prec, int nb, bool lds_links) {
+    Geom g = c->g;
+    if (c->lds_auto && !lds_links && prec == WL_F64 && (nb >= 4 || (nb >= 2 && g.vh >= (1 << 17)))) {
+        if (c->blk_auto) g.blk = 64;
+        g.lds = std::min(65536, 32768 * (g.blk / 64));
+    }
+    return g;
+}
+
 int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, const void *xin,
                   double a, int sign, int epi, int N, const int *all_done, CgState *st, void *x,
                   const void *p, void *r, int *pstride_out) {
@@ -275,7 +291,7 @@ int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, c
         L.p = p;
         L.r = r;
         L.partials = c->partials;
-        HIPCHK(launch_dslash_mr(prec, c->g, L, c->stream));
+        HIPCHK(launch_dslash_mr(prec, mr_geom(c, prec, L.NB, L.lds_links), L, c->stream));
     }
     if (pstride_out) *pstride_out = pstride;
     return WL_OK;
@@ -687,10 +703,10 @@ extern "C" int wl_set_option(wl_ctx *c, int option, int value) {
     switch (option) {
     case WL_OPT_TRAVERSAL_CHUNK: return set_chunk(c->g, value);
     case WL_OPT_BLOCK:
-        if (value == 0) value = dslash_block_auto(c->g.vh);
-        if (value != 64 && value != 128 && value != 256)
+        if (value != 0 && value != 64 && value != 128 && value != 256)
             return fail(WL_E_ARG, "workgroup size must be 64, 128 or 256");
-        c->g.blk = value;
+        c->blk_auto = value == 0;
+        c->g.blk = value == 0 ? dslash_block_auto(c->g.vh) : value;
         return WL_OK;
     case WL_OPT_RHS_BLOCK:
         if (value < 0 || value > 4) return fail(WL_E_ARG, "rhs per lane must be 0 (automatic) .. 4");
@@ -705,8 +721,9 @@ extern "C" int wl_set_option(wl_ctx *c, int option, int value) {
         c->link_load = value;
         return WL_OK;
     case WL_OPT_LDS_RESERVE:
-        if (value < 0 || value > 65536) return fail(WL_E_ARG, "LDS reserve must be 0..65536 bytes");
-        c->g.lds = value;
+        if (value < -1 || value > 65536)
+            return fail(WL_E_ARG, "LDS reserve must be -1 (none), 0 (automatic) or 1..65536 bytes");
+        set_lds_reserve(c, value);
         return WL_OK;
     case WL_OPT_GRAPHS:
         if (value != 0 && value != 1) return fail(WL_E_ARG, "graphs option must be 0 or 1");
@@ -739,7 +756,7 @@ extern "C" int wl_get_option(const wl_ctx *c, int option, int *value) {
     case WL_OPT_RHS_LAUNCH: *value = c->rhs_launch; return WL_OK;
     case WL_OPT_PLANE_PAD: *value = c->g.stride - plane_stride(c->g.vh, 0); return WL_OK;
     case WL_OPT_LINK_LOAD: *value = c->link_load; return WL_OK;
-    case WL_OPT_LDS_RESERVE: *value = c->g.lds; return WL_OK;
+    case WL_OPT_LDS_RESERVE: *value = lds_reserve_of(c); return WL_OK;
     case WL_OPT_GRAPHS: *value = c->use_graphs ? 1 : 0; return WL_OK;
     case WL_OPT_PIPELINE: *value = c->pipeline; return WL_OK;
     default: return fail(WL_E_ARG, "unknown option %d", option);
@@ -779,15 +796,28 @@ int time_hop_mr(wl_ctx *c, int prec, void *out, const void *in, int N, int iters
 }  // namespace
 
 namespace wl {
-bool tuned_config(const Geom &g, int prec, int nrhs, int *nr, int *nl, int *blk) {
+bool tuned_config(const Geom &g, int prec, int nrhs, wl_tune_result *r) {
     const TuneKey key{{g.L[0], g.L[1], g.L[2], g.L[3]}, prec, nrhs};
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find(key);
     if (it == g_tuned.end()) return false;
-    *nr = it->second.rhs_block;
-    *nl = it->second.rhs_launch;
-    *blk = it->second.block;
+    *r = it->second;
     return true;
+}
+
+void set_lds_reserve(wl_ctx *c, int value) {
+    c->lds_auto = value == 0;
+    c->g.lds = value > 0 ? value : 0;
+}
+
+int lds_reserve_of(const wl_ctx *c) { return c->lds_auto ? 0 : c->g.lds > 0 ? c->g.lds : -1; }
+
+void apply_hop_config(wl_ctx *c, const wl_tune_result &r) {
+    c->rhs_block = r.rhs_block;
+    c->rhs_launch = r.rhs_launch;
+    c->g.blk = r.block;
+    c->blk_auto = false;
+    set_lds_reserve(c, r.lds_reserve);
 }
 }  // namespace wl
 
@@ -806,9 +836,7 @@ extern "C" int wl_autotune(wl_ctx *c, int prec, int nrhs, wl_tune_result *out) {
         if (it != g_tuned.end()) {
             *out = it->second;
             out->candidates = 0;
-            c->rhs_block = out->rhs_block;
-            c->rhs_launch = out->rhs_launch;
-            c->g.blk = out->block;
+            apply_hop_config(c, *out);
             return WL_OK;
         }
     }
@@ -820,7 +848,16 @@ extern "C" int wl_autotune(wl_ctx *c, int prec, int nrhs, wl_tune_result *out) {
         return rc;
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    const int save_nr = c->rhs_block, save_nl = c->rhs_launch, save_blk = c->g.blk;
+    wl_tune_result saved{};
+    saved.rhs_block = c->rhs_block;
+    saved.rhs_launch = c->rhs_launch;
+    saved.block = c->g.blk;
+    saved.lds_reserve = lds_reserve_of(c);
+    const bool saved_blk_auto = c->blk_auto;
+    auto restore = [&]() {
+        apply_hop_config(c, saved);
+        c->blk_auto = saved_blk_auto;
+    };
     wl_tune_result best{};
     best.us_per_rhs = 1e300;
     int count = 0;
@@ -840,11 +877,17 @@ extern "C" int wl_autotune(wl_ctx *c, int prec, int nrhs, wl_tune_result *out) {
     c->rhs_block = 0;
     c->rhs_launch = 0;
     c->g.blk = dslash_block_auto(c->g.vh);
+    c->blk_auto = true;
+    set_lds_reserve(c, 0);
     double us_def = 0;
     if ((rc = time_hop_mr(c, prec, fout->data, fin->data, nrhs, iters, e0, e1, &us_def))) {
+        restore();
         cleanup();
         return rc;
     }
+    // LDS reserve: the caller's if set; else, in FP64, the automatic occupancy cap or none
+    std::vector<int> reserves{saved.lds_reserve};
+    if (saved.lds_reserve == 0 && prec == WL_F64) reserves.push_back(-1);
     const int nr_cap = prec == WL_F64 ? 2 : 4;
     static const int launches[] = {1, 2, 4, 6, 12, 16, 24, 32, 48, 64};
     for (int nr = 1; nr <= std::min(nr_cap, nrhs); nr *= 2) {
@@ -856,33 +899,31 @@ extern "C" int wl_autotune(wl_ctx *c, int prec, int nrhs, wl_tune_result *out) {
             if (want == nrhs) break;
         }
         for (int nl : nls)
-            for (int blk : {64, 128, 256}) {
-                c->rhs_block = nr;
-                c->rhs_launch = nl;
-                c->g.blk = blk;
-                double us = 0;
-                if ((rc = time_hop_mr(c, prec, fout->data, fin->data, nrhs, iters, e0, e1, &us))) {
-                    c->rhs_block = save_nr;
-                    c->rhs_launch = save_nl;
-                    c->g.blk = save_blk;
-                    cleanup();
-                    return rc;
+            for (int blk : {64, 128, 256})
+                for (int lds : reserves) {
+                    wl_tune_result cand{};
+                    cand.rhs_block = nr;
+                    cand.rhs_launch = nl;
+                    cand.block = blk;
+                    cand.lds_reserve = lds;
+                    apply_hop_config(c, cand);
+                    double us = 0;
+                    if ((rc = time_hop_mr(c, prec, fout->data, fin->data, nrhs, iters, e0, e1, &us))) {
+                        restore();
+                        cleanup();
+                        return rc;
+                    }
+                    ++count;
+                    if (us < best.us_per_rhs * nrhs) {
+                        cand.us_per_rhs = us / nrhs;
+                        best = cand;
+                    }
                 }
-                ++count;
-                if (us < best.us_per_rhs * nrhs) {
-                    best.rhs_block = nr;
-                    best.rhs_launch = nl;
-                    best.block = blk;
-                    best.us_per_rhs = us / nrhs;
-                }
-            }
     }
     cleanup();
     best.candidates = count;
     best.us_per_rhs_default = us_def / nrhs;
-    c->rhs_block = best.rhs_block;
-    c->rhs_launch = best.rhs_launch;
-    c->g.blk = best.block;
+    apply_hop_config(c, best);
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         g_tuned.emplace(key, best);

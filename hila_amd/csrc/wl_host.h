@@ -87,7 +87,10 @@ int meo_mr(wl_ctx *c, int prec, void *out, const void *in, void *t, double kappa
 
 // The configuration wl_autotune chose for (lattice size, precision, nrhs) in this process, if
 // any: rhs per lane, rhs per launch, workgroup size.
-bool tuned_config(const Geom &g, int prec, int nrhs, int *nr, int *nl, int *blk);
+bool tuned_config(const Geom &g, int prec, int nrhs, wl_tune_result *r);
+void apply_hop_config(wl_ctx *c, const wl_tune_result &r);   // rhs block/launch, block, LDS
+void set_lds_reserve(wl_ctx *c, int value);                  // WL_OPT_LDS_RESERVE semantics
+int lds_reserve_of(const wl_ctx *c);                         // ... and its current value
 
 // Upload / download of right-hand side j of an N-rhs field from / to host AoS spinors
 // (V/2 x 24 doubles, EVEN_SITES_FIRST order within the parity), enqueued on the context's

@@ -204,6 +204,8 @@ struct wl_ctx {
     int link_load = 0;                 // multi-rhs links: 0 automatic, 1 cached, 2 nontemporal, 3 LDS
     bool use_graphs = false;           // batched CG: replay each chunk from a captured HIP graph
     int pipeline = 0;                  // multi-rhs hops: 0 pipelined where possible, 1 never
+    bool blk_auto = true;              // g.blk is the volume's default (not chosen by a caller)
+    bool lds_auto = true;              // g.lds unset: multi-rhs hops take mr_geom's occupancy cap
     double op_shift = 0.0;             // Hasenbusch shift mh of the operator being applied,
                                        // M + mh (0 outside a shifted call; see OpShift)
     // fields created through wl_field_create and not yet destroyed; wl_destroy orphans them

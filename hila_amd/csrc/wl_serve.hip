@@ -331,11 +331,16 @@ void scheduler_main(wl_server *S) {
             // kernel configuration: the autotuned one for this shape if wl_autotune chose one
             // in this process, else the defaults (never timed here)
             wl_ctx *c = s->ctx;
-            int nr = 0, nl = 0, blk = dslash_block_auto(c->g.vh);
-            (void)tuned_config(c->g, r.inner_prec, r.nrhs, &nr, &nl, &blk);
-            c->rhs_block = nr;
-            c->rhs_launch = nl;
-            c->g.blk = blk;
+            wl_tune_result tuned{};
+            if (tuned_config(c->g, r.inner_prec, r.nrhs, &tuned)) {
+                apply_hop_config(c, tuned);
+            } else {
+                c->rhs_block = 0;
+                c->rhs_launch = 0;
+                c->g.blk = dslash_block_auto(c->g.vh);
+                c->blk_auto = true;
+                set_lds_reserve(c, 0);
+            }
             CgSpec spec;
             spec.kappa = r.kappa;
             spec.mh = r.mh;

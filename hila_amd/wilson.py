@@ -87,7 +87,8 @@ class SolveResult(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("rhs_block", C.c_int), ("rhs_launch", C.c_int), ("block", C.c_int),
-                ("candidates", C.c_int), ("us_per_rhs", C.c_double), ("us_per_rhs_default", C.c_double)]
+                ("candidates", C.c_int), ("us_per_rhs", C.c_double), ("us_per_rhs_default", C.c_double),
+                ("lds_reserve", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

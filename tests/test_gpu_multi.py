@@ -78,6 +78,53 @@ def test_rhs_block_and_workgroup_options_do_not_change_results():
         lat.set_option(W.OPT_RHS_BLOCK, 5)
 
 
+def test_fp64_occupancy_cap_and_lds_reserve_do_not_change_results():
+    """2^17 sites per parity: FP64 multi-rhs hops take the automatic occupancy cap (64-lane
+    workgroups, 32 KB LDS each); no reserve, an explicit one and a chosen workgroup size give
+    the same bits."""
+    dims, N = (16, 16, 32, 32), 3
+    lat, U = _setup(dims)
+    del U
+    D = W.Dirac_Wilson_evenodd(0.125, lat)
+    fin = W.Field(lat, EVEN, W.F64, nrhs=N)
+    for j in range(N):
+        fin.upload(W.gen_spinor(dims, 450 + j, EVEN), rhs=j)
+    fe = W.Field(lat, EVEN, W.F64, nrhs=N)
+    assert lat.get_option(W.OPT_LDS_RESERVE) == 0
+    ref = None
+    for lds in (0, -1, 32768, 0):
+        for blk in (0, 128):
+            lat.set_option(W.OPT_LDS_RESERVE, lds)
+            lat.set_option(W.OPT_BLOCK, blk)
+            assert lat.get_option(W.OPT_LDS_RESERVE) == lds
+            D.apply(fin, fe)
+            out = fe.download().tobytes()
+            if ref is None:
+                ref = out
+            assert out == ref, (lds, blk)
+    for bad in (-2, 65537):
+        with pytest.raises(W.WilsonError):
+            lat.set_option(W.OPT_LDS_RESERVE, bad)
+    lat.close()
+
+
+def test_autotune_remembers_its_choice_including_the_lds_reserve():
+    dims, N = (8, 8, 8, 8), 2
+    lat, U = _setup(dims)
+    del U
+    first = lat.autotune(W.F64, N)
+    assert first["candidates"] > 0 and first["lds_reserve"] in (0, -1)
+    assert lat.get_option(W.OPT_LDS_RESERVE) == first["lds_reserve"]
+    assert lat.get_option(W.OPT_BLOCK) == first["block"]
+    lat.set_option(W.OPT_LDS_RESERVE, 4096)
+    again = lat.autotune(W.F64, N)
+    assert again["candidates"] == 0
+    assert {k: again[k] for k in ("rhs_block", "rhs_launch", "block", "lds_reserve")} == \
+        {k: first[k] for k in ("rhs_block", "rhs_launch", "block", "lds_reserve")}
+    assert lat.get_option(W.OPT_LDS_RESERVE) == first["lds_reserve"]
+    lat.close()
+
+
 def test_batched_f64_cg_matches_single_rhs_solves():
     dims, N = (8, 8, 8, 8), 4
     lat, U = _setup(dims, "haar", 0, 1)
