@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0            # MI355X HBM3E, 8 TB/s
 BYTES_PER_SITE_F64 = 1536        # 8*144 + 192 + 192
 FLOPS_PER_SITE = 1320
+BYTES_NO_REUSE_F64 = 2880        # 8*(24+18) + 24 reals: every neighbour spinor read per hop
 DIMS = (32, 32, 32, 32)
 KAPPA = 0.125
 METRIC = "Dslash GB/s & %HBM roofline; CG solves/s + p99 latency @1e-10 true residual"
@@ -70,6 +71,23 @@ def solve_series(W, D, fin, fx, n, precision, accuracy=1e-10, delta=1e-3):
         "precision": precision,
         "solves": len(lat_ms),
     }
+
+
+SMALL_DIMS, SMALL_KAPPA = (8, 8, 8, 8), 0.12
+
+
+def small_cg_config(W, n):
+    """The plain method's CPU-runnable configuration on the GPU: 8^4, Haar links (seed 1),
+    kappa 0.12, Gaussian source (seed 2), FP64 CG to 1e-10.  3 MB of links and spinors sit in
+    the caches, so this measures launch latency, not bandwidth."""
+    U = W.gen_gauge(SMALL_DIMS, "haar", 0, 1)
+    lat = W.Lattice(SMALL_DIMS)
+    lat.upload_gauge(U)
+    D = W.Dirac_Wilson_evenodd(SMALL_KAPPA, lat)
+    fin = W.Field(lat, W.EVEN).upload(W.gen_spinor(SMALL_DIMS, 2, W.EVEN))
+    out = solve_series(W, D, fin, W.Field(lat, W.EVEN), max(n, 5), "f64")
+    lat.close()
+    return out
 
 
 def mixed_precision_config(W, n):
@@ -575,21 +593,45 @@ def hmc_solver_config(W, nbasis=4, nseq=7, mh=0.1, noise=1e-3, kappa=0.124, tol=
     return out
 
 
-def cpu_baseline(U, psi, seconds=15.0):
+def cpu_baseline(W, U, psi, seconds=8.0):
     """The oracle's HILA-structured hopping term (projection pass + per-direction
-    accumulation, wilson.h:53-95) on the host cores, on a bounded number of applications."""
-    from oracle.oracle import ODD, Oracle
-    orc = Oracle(DIMS)
-    orc.hop_set(U, KAPPA, psi, ODD, 1)  # warm-up (page-in, thread pool)
-    n, t0 = 0, time.perf_counter()
-    while True:
-        orc.hop_set(U, KAPPA, psi, ODD, 1)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or n >= 200:
-            break
+    accumulation, wilson.h:53-95) on the host cores, on a bounded number of applications;
+    then the same on one core, and the oracle's CG (conjugate_gradient.h:46-104) to 1e-10 at
+    8^4 (Haar seed 1, kappa 0.12, source seed 2) and at 32^4 (the Dslash's field and source)."""
+    from oracle import oracle as O
+    orc = O.Oracle(DIMS)
+    orc.hop_set(U, KAPPA, psi, O.ODD, 1)  # warm-up (page-in, thread pool)
+
+    def sample(limit_s, limit_n):
+        n, t0 = 0, time.perf_counter()
+        while True:
+            orc.hop_set(U, KAPPA, psi, O.ODD, 1)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= limit_s or n >= limit_n:
+                return n, el
+
+    n, el = sample(seconds, 200)
     vh = orc.Vh
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = O.set_threads(0)
+    O.set_threads(1)
+    n1, el1 = sample(3.0, 20)
+    O.set_threads(cores)
+
+    def cg(dims, Ug, kappa, src):
+        o = O.Oracle(dims)
+        t0 = time.perf_counter()
+        x, steps, _ = o.cg(Ug, kappa, src, accuracy=1e-10)
+        ms = 1e3 * (time.perf_counter() - t0)
+        b = src.copy()
+        b[o.Vh:] = 0
+        r = b - o.meo(Ug, kappa, o.meo(Ug, kappa, x, 1), -1)
+        return {"ms_per_solve": round(ms, 1), "iterations": int(steps),
+                "true_resid": float(np.linalg.norm(r) / np.linalg.norm(b))}
+
+    small = cg(SMALL_DIMS, W.gen_gauge(SMALL_DIMS, "haar", 0, 1), SMALL_KAPPA,
+               W.gen_spinor(SMALL_DIMS, 2, W.EVEN))
+    big = cg(DIMS, U, KAPPA, psi)
     return {
         "value": round(BYTES_PER_SITE_F64 * vh * n / el / 1e9, 3),
         "unit": "GB/s",
@@ -598,6 +640,11 @@ def cpu_baseline(U, psi, seconds=15.0):
         "sample": f"{n} half-lattice hopping applications (Dirac_Wilson_hop_set restated, "
                   f"HILA pass structure) at 32^4 FP64, {el:.1f} s, OpenMP {cores} threads",
         "ms_per_apply": round(1e3 * el / n, 3),
+        "single_core": {"value": round(BYTES_PER_SITE_F64 * vh * n1 / el1 / 1e9, 3),
+                        "unit": "GB/s", "ms_per_apply": round(1e3 * el1 / n1, 1),
+                        "sample": f"{n1} applications, {el1:.1f} s, one thread"},
+        "cg_8x8x8x8": small,
+        "cg_32x32x32x32": big,
     }
 
 
@@ -697,6 +744,12 @@ def main(argv=None):
     # the kernel's own average duration, HIP events on the library's stream
     ms_kernel = D.time(fin, fodd, "hop", 1, warmup=5, iters=max(a.steps, 100))
     ms_M = D.time(fin, W.Field(lat, W.EVEN), "M", 1, warmup=3, iters=50)
+    # device-copy bandwidth on the same byte count: one 4-rhs FP64 even field (402 MB) copied
+    # into another, 805 MB moved, as many bytes as one hopping application
+    ca, cb = W.Field(lat, W.EVEN, W.F64, 4), W.Field(lat, W.EVEN, W.F64, 4)
+    ms_copy = D.time(ca, cb, "copy", warmup=5, iters=50)
+    copy_bytes = 2 * 4 * 192 * vh
+    del ca, cb
     alg_bytes = BYTES_PER_SITE_F64 * vh
     gbs_wall = alg_bytes / (ms_per_step * 1e-3) / 1e9
     gbs_kernel = alg_bytes / (ms_kernel * 1e-3) / 1e9
@@ -717,6 +770,7 @@ def main(argv=None):
         cg_mixed["speedup_vs_f64"] = round(cg_mixed["solves_per_s"] / cg_out["solves_per_s"], 3)
         cg_mixed["solution_rel_diff_vs_f64"] = rel_diff(xmx.download(), x64.download())
     del x64, xmx
+    small = small_cg_config(W, a.cg_solves) if a.cg_solves > 0 else None
     big = multi = serve = None
     D = fin = fodd = None
     lat.close()
@@ -756,11 +810,15 @@ def main(argv=None):
                      "unit": "GB/s", "frac": round(gbs_kernel / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "k_dslash<double,+1,EPI_SET>",
                      "kernel_us": round(ms_kernel * 1e3, 2),
-                     "algorithmic_bytes_per_launch": alg_bytes},
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "copy_gbs": round(copy_bytes / (ms_copy * 1e-3) / 1e9, 1),
+                     "frac_of_copy": round(gbs_kernel / (copy_bytes / (ms_copy * 1e-3) / 1e9), 4)},
+        "effective_gbs_no_reuse": round(BYTES_NO_REUSE_F64 * vh / (ms_kernel * 1e-3) / 1e9, 1),
         "gflops": round(FLOPS_PER_SITE * vh / (ms_kernel * 1e-3) / 1e9, 1),
         "meo_us": round(ms_M * 1e3, 2),
         "cg": cg_out,
         "cg_mixed_f32": cg_mixed,
+        "cg_8x8x8x8": small,
         "mixed_48x48x48x96": big,
         "multi_rhs_32x32x32x32": multi,
         "serving": serve,
@@ -770,7 +828,7 @@ def main(argv=None):
     }
     if not a.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(U, np.ascontiguousarray(psi))
+        out["cpu_baseline"] = cpu_baseline(W, U, np.ascontiguousarray(psi))
     print(json.dumps(out), flush=True)
 
 

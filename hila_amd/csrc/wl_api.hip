@@ -671,9 +671,11 @@ extern "C" int wl_time_op(wl_ctx *c, int what, wl_field *out, const wl_field *in
     if (!c || !out || !in || !ms_per_launch || iters < 1 || warmup < 0)
         return fail(WL_E_ARG, "bad argument");
     auto once = [&]() -> int {
+        if (what == 2) return wl_field_copy(out, in);
         return what == 0 ? wl_dslash_eo(c, out, in, kappa, sign) : wl_meo_apply(c, out, in, kappa, sign);
     };
-    if (what != 0 && what != 1) return fail(WL_E_ARG, "what must be 0 (hopping term) or 1 (M)");
+    if (what < 0 || what > 2) return fail(WL_E_ARG, "what must be 0 (hopping term), 1 (M) or 2 (copy)");
+    if (what == 2 && (out->ctx != c || in->ctx != c)) return fail(WL_E_ARG, "fields of another context");
     for (int i = 0; i < warmup; ++i)
         if (int rc = once()) return rc;
     DeviceGuard dg(c->device);

@@ -51,35 +51,42 @@ struct RbArgs {
     int pstride;               // partial sums: partials[rhs * pstride + segment]
 };
 
+// The projected half spinor of a stored neighbour spinor in FP32 or FP64 storage, widened
+// first (FP16 storage is projected packed, hop_pk).
+template <int MU, int S, typename T>
+__device__ __forceinline__ void rb_project(const typename St<T>::E (&wr)[12], cx<typename St<T>::R> (&h)[2][3]) {
+    cx<typename St<T>::R> w[4][3];
+#pragma unroll
+    for (int c = 0; c < 12; ++c) w[c / 3][c % 3] = widen(wr[c]);
+    project<MU, S>(w, h);
+}
+
+// acc += f expand((1 + S gamma_mu) w, U or U^dag) for one rhs in FP16 storage: packed
+// projection and U h (wl_store.h), FP32 accumulation; f = site scale x antiperiodic sign.
+template <int MU, int SIGN, bool FWD>
+__device__ __forceinline__ void hop_pk(cx<float> (&acc)[4][3], const h2 (&w)[12], const h2 (&u)[9], float f) {
+    h2 hr[2][3], hi[2][3];
+    cx<float> v[2][3];
+    project_pk<MU, FWD ? SIGN : -SIGN, FWD>(w, hr, hi);
+    su3_mul_pk<FWD>(u, hr, hi, v);
+    expand_add_scaled<MU, FWD ? SIGN : -SIGN>(acc, v, f);
+}
+
 // One hop for the NR right-hand sides of this lane: link once, NR spinors.  NTL: load the
 // link with nontemporal hints (it is not read again by this launch), so that the link stream
 // does not displace the neighbour spinors from the caches.  For the scaled format the site
 // scale of each neighbour is applied to the product U h (6 complex numbers) instead of to the
 // 12 components of the spinor: the hop is linear.
-// The projected half spinor of a stored neighbour spinor: FP16 storage straight from the halves
-// (project_h2), the other formats widened first.
-template <int MU, int S, typename T>
-__device__ __forceinline__ void rb_project(const typename St<T>::E (&wr)[12], cx<typename St<T>::R> (&h)[2][3]) {
-    if constexpr (St<T>::scaled) {
-        project_h2<MU, S>(wr, h);
-    } else {
-        cx<typename St<T>::R> w[4][3];
-#pragma unroll
-        for (int c = 0; c < 12; ++c) w[c / 3][c % 3] = widen(wr[c]);
-        project<MU, S>(w, h);
-    }
-}
-
 template <int MU, int SIGN, bool FWD, int NR, bool NTL, typename T>
 __device__ __forceinline__ void rb_hop(cx<typename St<T>::R> (&acc)[NR][4][3], const bool (&act)[NR],
                                        const typename St<T>::E *__restrict__ in, size_t rs,
                                        const typename St<T>::E *__restrict__ link, int S, int kn,
                                        int kl, typename St<T>::R sg) {
     using R = typename St<T>::R;
+    typename St<T>::E ur[9];
+    ld_link_raw<NTL, T>(ur, link, S, kl);
     cx<R> u[3][3];
-    {
-        typename St<T>::E ur[9];
-        ld_link_raw<NTL, T>(ur, link, S, kl);
+    if constexpr (!St<T>::scaled) {
 #pragma unroll
         for (int e = 0; e < 9; ++e) u[e / 3][e % 3] = widen(ur[e]);
     }
@@ -94,27 +101,26 @@ __device__ __forceinline__ void rb_hop(cx<typename St<T>::R> (&acc)[NR][4][3], c
             else
                 sc[r] = sg;
         }
+    if constexpr (St<T>::scaled) {
 #pragma unroll
-    for (int r = 0; r < NR; ++r)
-        if (act[r]) {
-            cx<R> h[2][3], v[2][3];
-            rb_project<MU, FWD ? SIGN : -SIGN, T>(wr[r], h);
-            if constexpr (FWD)
-                su3_mul(u, h, v);
-            else
-                su3_adj_mul(u, h, v);
-            if constexpr (MU == 3 || St<T>::scaled) {
+        for (int r = 0; r < NR; ++r)
+            if (act[r]) hop_pk<MU, SIGN, FWD>(acc[r], wr[r], ur, sc[r]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (act[r]) {
+                cx<R> h[2][3], v[2][3];
+                rb_project<MU, FWD ? SIGN : -SIGN, T>(wr[r], h);
                 if constexpr (FWD)
-                    expand_add_scaled<MU, SIGN>(acc[r], v, sc[r]);
+                    su3_mul(u, h, v);
                 else
-                    expand_add_scaled<MU, -SIGN>(acc[r], v, sc[r]);
-            } else {
-                if constexpr (FWD)
-                    expand_add<MU, SIGN>(acc[r], v);
+                    su3_adj_mul(u, h, v);
+                if constexpr (MU == 3)
+                    expand_add_scaled<MU, FWD ? SIGN : -SIGN>(acc[r], v, sc[r]);
                 else
-                    expand_add<MU, -SIGN>(acc[r], v);
+                    expand_add<MU, FWD ? SIGN : -SIGN>(acc[r], v);
             }
-        }
+    }
 }
 
 // Software-pipelined form of the 8 hops (PIPE): hop h+1's link and NR neighbour spinors are
@@ -148,7 +154,8 @@ __device__ __forceinline__ void rb_load(RbBuf<T, NR> &b, const bool (&act)[NR],
         }
 }
 
-// acc += the hop from the buffered spinors and the link u; sg: antiperiodic sign (t hops only)
+// acc += the hop from the buffered spinors and the link u (FP32 or FP64 storage); sg:
+// antiperiodic sign (t hops only)
 template <int MU, int SIGN, bool FWD, int NR, typename T>
 __device__ __forceinline__ void rb_compute_u(cx<typename St<T>::R> (&acc)[NR][4][3], const RbBuf<T, NR> &b,
                                              const cx<typename St<T>::R> (&u)[3][3], const bool (&act)[NR],
@@ -163,19 +170,10 @@ __device__ __forceinline__ void rb_compute_u(cx<typename St<T>::R> (&acc)[NR][4]
                 su3_mul(u, h, v);
             else
                 su3_adj_mul(u, h, v);
-            if constexpr (MU == 3 || St<T>::scaled) {
-                R f = MU == 3 ? sg : R(1);
-                if constexpr (St<T>::scaled) f *= b.sc[r];
-                if constexpr (FWD)
-                    expand_add_scaled<MU, SIGN>(acc[r], v, f);
-                else
-                    expand_add_scaled<MU, -SIGN>(acc[r], v, f);
-            } else {
-                if constexpr (FWD)
-                    expand_add<MU, SIGN>(acc[r], v);
-                else
-                    expand_add<MU, -SIGN>(acc[r], v);
-            }
+            if constexpr (MU == 3)
+                expand_add_scaled<MU, FWD ? SIGN : -SIGN>(acc[r], v, sg);
+            else
+                expand_add<MU, FWD ? SIGN : -SIGN>(acc[r], v);
         }
 }
 
@@ -184,12 +182,18 @@ template <int MU, int SIGN, bool FWD, int NR, typename T>
 __device__ __forceinline__ void rb_compute(cx<typename St<T>::R> (&acc)[NR][4][3], const RbBuf<T, NR> &b,
                                            const bool (&act)[NR], typename St<T>::R sg) {
     using R = typename St<T>::R;
-    cx<R> u[3][3];
+    if constexpr (St<T>::scaled) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+        for (int r = 0; r < NR; ++r)
+            if (act[r]) hop_pk<MU, SIGN, FWD>(acc[r], b.w[r], b.u, (MU == 3 ? sg : R(1)) * b.sc[r]);
+    } else {
+        cx<R> u[3][3];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) u[i][j] = widen(b.u[i * 3 + j]);
-    rb_compute_u<MU, SIGN, FWD, NR, T>(acc, b, u, act, sg);
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) u[i][j] = widen(b.u[i * 3 + j]);
+        rb_compute_u<MU, SIGN, FWD, NR, T>(acc, b, u, act, sg);
+    }
 }
 
 // Hop H = 2 mu + (0 forward, 1 backward): issue hop H+1's loads, then compute hop H.
@@ -373,10 +377,20 @@ __device__ __forceinline__ void rb_chain_lds(cx<typename St<T>::R> (&acc)[NR][4]
         } else {
             sp_load<NR, T>(buf[H & 1], act, in, rs, S, (H % 2 == 0) ? n.fwd[H / 2] : n.bwd[H / 2]);
         }
-        cx<R> u[3][3];
+        if constexpr (St<T>::scaled) {
+            h2 ur[9];
 #pragma unroll
-        for (int e = 0; e < 9; ++e) u[e / 3][e % 3] = widen(lu[(H * 9 + e) * 64 + lane]);
-        rb_compute_u<H / 2, SIGN, (H % 2 == 0), NR, T>(acc, buf[H & 1], u, act, (H % 2 == 0) ? sf : sb);
+            for (int e = 0; e < 9; ++e) ur[e] = lu[(H * 9 + e) * 64 + lane];
+            const R sg = (H / 2 == 3) ? ((H % 2 == 0) ? sf : sb) : R(1);
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (act[r]) hop_pk<H / 2, SIGN, (H % 2 == 0)>(acc[r], buf[H & 1].w[r], ur, sg * buf[H & 1].sc[r]);
+        } else {
+            cx<R> u[3][3];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) u[e / 3][e % 3] = widen(lu[(H * 9 + e) * 64 + lane]);
+            rb_compute_u<H / 2, SIGN, (H % 2 == 0), NR, T>(acc, buf[H & 1], u, act, (H % 2 == 0) ? sf : sb);
+        }
         rb_chain_lds<T, SIGN, H + 1, NR, PIPE>(acc, buf, act, in, rs, lu, lane, S, n, sf, sb);
     }
 }

@@ -230,56 +230,96 @@ __device__ __forceinline__ void st_site(typename St<T>::E *base, int S, int k,
     }
 }
 
-// ------------------------------------------- spin projection straight from FP16 storage ----
-// half(a, SA) + (NEG ? -1 : 1) half(b, SB) in FP32, where half(v, 0 / 1) is the low (real) /
-// high (imaginary) FP16 half of a stored complex number: one v_fma_mix_f32 (a * 1.0 + b with
-// both FP16 operands widened inside the instruction, one rounding) instead of two conversions
-// and an addition.  The compiler does not form it from (float)a + (float)b by itself.  Against
-// the convert-then-add form the hop's outputs were measured to differ on under 2% of the
-// components, by one FP32 rounding (and so at most one FP16 step where they are stored).
-template <int SA, int SB, bool NEG>
-__device__ __forceinline__ float mix_add(h2 a, h2 b) {
+// ------------------------------------ packed FP16 hop arithmetic, FP32 accumulation ------
+// The FP16 hop keeps links and spinors packed as stored: the spin projection is one
+// v_pk_add_f16 per complex number and operand form (one FP16 rounding of each half, the same
+// size as the storage rounding of the operands), and each complex multiply-add of U h is two
+// v_dot2_f32_f16 (products of halves, summed into an FP32 accumulator) instead of four FP32
+// FMAs after widening.  The projection is written as VALU assembly because the compiler does
+// not fold the half selections and negations into v_pk_add_f16's modifiers (op_sel /
+// op_sel_hi pick the half of each source used for the low / high result, neg_lo / neg_hi
+// negate it); the dot products are compiler builtins, so that the compiler sees them and keeps
+// the wait states a dot-product result needs before another VALU instruction reads it.
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// a + C b for stored complex numbers, C = +1 (1), -1 (-1), +i (2) or -i (-2), as the packed
+// operand of the dot products, in form F of the result z = (re, im):
+//   F = 0: (re, im)   1: (im, re)   2: (re, -im)   3: (im, -re)
+template <int C, int F>
+__device__ __forceinline__ h2 pk_cadd(h2 a, h2 b) {
     const unsigned ua = __builtin_bit_cast(unsigned, a), ub = __builtin_bit_cast(unsigned, b);
-    float r;
-#define WL_MIX(SEL, NEGB)                                                                       \
-    asm("v_fma_mix_f32 %0, %1, 1.0, " NEGB "%2 op_sel:" SEL " op_sel_hi:[1,0,1]" : "=v"(r)      \
-        : "v"(ua), "v"(ub))
-    if constexpr (SA == 0 && SB == 0) {
-        if constexpr (NEG) WL_MIX("[0,0,0]", "-"); else WL_MIX("[0,0,0]", "");
-    } else if constexpr (SA == 0 && SB == 1) {
-        if constexpr (NEG) WL_MIX("[0,0,1]", "-"); else WL_MIX("[0,0,1]", "");
-    } else if constexpr (SA == 1 && SB == 0) {
-        if constexpr (NEG) WL_MIX("[1,0,0]", "-"); else WL_MIX("[1,0,0]", "");
-    } else {
-        if constexpr (NEG) WL_MIX("[1,0,1]", "-"); else WL_MIX("[1,0,1]", "");
+    unsigned d;
+#define WL_PK(MODS) asm("v_pk_add_f16 %0, %1, %2 " MODS : "=v"(d) : "v"(ua), "v"(ub))
+    // same halves (C = +-1) or crossed (C = +-i); swapped result (F odd); negated high half
+    // (F >= 2): neg_hi flipped for both sources
+    if constexpr (C == 1) {
+        if constexpr (F == 0) WL_PK("");
+        else if constexpr (F == 1) WL_PK("op_sel:[1,1] op_sel_hi:[0,0]");
+        else if constexpr (F == 2) WL_PK("neg_hi:[1,1]");
+        else WL_PK("op_sel:[1,1] op_sel_hi:[0,0] neg_hi:[1,1]");
+    } else if constexpr (C == -1) {
+        if constexpr (F == 0) WL_PK("neg_lo:[0,1] neg_hi:[0,1]");
+        else if constexpr (F == 1) WL_PK("op_sel:[1,1] op_sel_hi:[0,0] neg_lo:[0,1] neg_hi:[0,1]");
+        else if constexpr (F == 2) WL_PK("neg_lo:[0,1] neg_hi:[1,0]");
+        else WL_PK("op_sel:[1,1] op_sel_hi:[0,0] neg_lo:[0,1] neg_hi:[1,0]");
+    } else if constexpr (C == 2) {   // z = (a.re - b.im, a.im + b.re)
+        if constexpr (F == 0) WL_PK("op_sel:[0,1] op_sel_hi:[1,0] neg_lo:[0,1]");
+        else if constexpr (F == 1) WL_PK("op_sel:[1,0] op_sel_hi:[0,1] neg_hi:[0,1]");
+        else if constexpr (F == 2) WL_PK("op_sel:[0,1] op_sel_hi:[1,0] neg_lo:[0,1] neg_hi:[1,1]");
+        else WL_PK("op_sel:[1,0] op_sel_hi:[0,1] neg_hi:[1,0]");
+    } else {                         // z = (a.re + b.im, a.im - b.re)
+        if constexpr (F == 0) WL_PK("op_sel:[0,1] op_sel_hi:[1,0] neg_hi:[0,1]");
+        else if constexpr (F == 1) WL_PK("op_sel:[1,0] op_sel_hi:[0,1] neg_lo:[0,1]");
+        else if constexpr (F == 2) WL_PK("op_sel:[0,1] op_sel_hi:[1,0] neg_hi:[1,0]");
+        else WL_PK("op_sel:[1,0] op_sel_hi:[0,1] neg_lo:[0,1] neg_hi:[1,1]");
     }
-#undef WL_MIX
-    return r;
+#undef WL_PK
+    return __builtin_bit_cast(h2, d);
 }
 
-// a + c b for stored complex numbers, c = +1 (C == 1), -1 (-1), +i (2) or -i (-2)
-template <int C>
-__device__ __forceinline__ cx<float> mix_cadd(h2 a, h2 b) {
-    if constexpr (C == 1) return {mix_add<0, 0, false>(a, b), mix_add<1, 1, false>(a, b)};
-    else if constexpr (C == -1) return {mix_add<0, 0, true>(a, b), mix_add<1, 1, true>(a, b)};
-    else if constexpr (C == 2) return {mix_add<0, 1, true>(a, b), mix_add<1, 0, false>(a, b)};
-    else return {mix_add<0, 1, false>(a, b), mix_add<1, 0, true>(a, b)};
+// u.x v.x + u.y v.y + c in FP32 (v_dot2_f32_f16)
+__device__ __forceinline__ float dot2(h2 u, h2 v, float c) {
+    return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, u), __builtin_bit_cast(half2_t, v), c, false);
 }
 
-// project<MU, S> (wl_device.h) of a spinor held in FP16 storage (12 stored complex numbers,
-// component c = spin * 3 + colour), without widening it first.
-template <int MU, int S>
-__device__ __forceinline__ void project_h2(const h2 (&w)[12], cx<float> (&h)[2][3]) {
-    // h0 = w0 + c0 w[b0], h1 = w1 + c1 w[b1]: the coefficients of wl_device.h's project
+// project<MU, S> (wl_device.h) of a spinor in FP16 storage, kept packed, as the two operands
+// of U h (FWD: forms 2 and 1) or of U^dag h (forms 0 and 3):
+//   U h:      re = sum u.re h.re - u.im h.im = dot(u, (re, -im)),  im = dot(u, (im, re))
+//   U^dag h:  re = sum u.re h.re + u.im h.im = dot(u, (re, im)),   im = dot(u, (im, -re))
+// with u = U[i][j] for U h and u = U[j][i] for U^dag h.
+template <int MU, int S, bool FWD>
+__device__ __forceinline__ void project_pk(const h2 (&w)[12], h2 (&hr)[2][3], h2 (&hi)[2][3]) {
     constexpr int b0 = (MU == 0 || MU == 1) ? 3 : 2;
     constexpr int b1 = (MU == 0 || MU == 1) ? 2 : 3;
     constexpr int c0 = MU == 0 ? 2 * S : MU == 1 ? -S : MU == 2 ? 2 * S : S;
     constexpr int c1 = MU == 0 ? 2 * S : MU == 1 ? S : MU == 2 ? -2 * S : S;
+    constexpr int FR = FWD ? 2 : 0, FI = FWD ? 1 : 3;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        h[0][c] = mix_cadd<c0>(w[c], w[b0 * 3 + c]);
-        h[1][c] = mix_cadd<c1>(w[3 + c], w[b1 * 3 + c]);
+        hr[0][c] = pk_cadd<c0, FR>(w[c], w[b0 * 3 + c]);
+        hi[0][c] = pk_cadd<c0, FI>(w[c], w[b0 * 3 + c]);
+        hr[1][c] = pk_cadd<c1, FR>(w[3 + c], w[b1 * 3 + c]);
+        hi[1][c] = pk_cadd<c1, FI>(w[3 + c], w[b1 * 3 + c]);
     }
+}
+
+// v = U h (FWD) or U^dag h for a link of 9 stored entries (row-major), from project_pk's
+// operands.
+template <bool FWD>
+__device__ __forceinline__ void su3_mul_pk(const h2 (&u)[9], const h2 (&hr)[2][3], const h2 (&hi)[2][3],
+                                           cx<float> (&v)[2][3]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const h2 u0 = FWD ? u[i * 3 + 0] : u[0 * 3 + i];
+            const h2 u1 = FWD ? u[i * 3 + 1] : u[1 * 3 + i];
+            const h2 u2 = FWD ? u[i * 3 + 2] : u[2 * 3 + i];
+            const float re = dot2(u2, hr[s][2], dot2(u1, hr[s][1], dot2(u0, hr[s][0], 0.f)));
+            const float im = dot2(u2, hi[s][2], dot2(u1, hi[s][1], dot2(u0, hi[s][0], 0.f)));
+            v[s][i] = {re, im};
+        }
 }
 
 // Host side: precision code -> planes per rhs and bytes per stored element.

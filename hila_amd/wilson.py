@@ -414,9 +414,11 @@ class Dirac_Wilson_evenodd:
 
     def time(self, fin: Field, fout: Field, what="hop", sign=1, warmup=3, iters=100):
         """Average device milliseconds per call (HIP events on the library's stream) of the
-        hopping term (what='hop') or of M (what='M')."""
+        hopping term (what='hop'), of M (what='M') or of a plain copy fout = fin
+        (what='copy')."""
         ms = C.c_double(0)
-        check(lib().wl_time_op(self.lat.handle, 0 if what == "hop" else 1, fout.handle,
+        code = {"hop": 0, "M": 1, "copy": 2}[what]
+        check(lib().wl_time_op(self.lat.handle, code, fout.handle,
                                fin.handle, self.kappa, sign, warmup, iters, C.byref(ms)))
         return ms.value
 

@@ -342,7 +342,9 @@ int wl_autotune(wl_ctx *ctx, int prec, int nrhs, wl_tune_result *out);
 
 /* Average device time per call of `iters` back-to-back calls (after `warmup` untimed ones),
  * measured with HIP events recorded on the context's stream, around the hopping term
- * (what = 0, as wl_dslash_eo) or M (what = 1, as wl_meo_apply).  No counterpart in HILA
+ * (what = 0, as wl_dslash_eo), M (what = 1, as wl_meo_apply) or a plain copy out = in of two
+ * fields of the context (what = 2, as wl_field_copy; kappa and sign unused), the device-copy
+ * bandwidth the hopping term is compared with.  No counterpart in HILA
  * (its benchmark times the host loop, applications/benchmarks/bench_fermion.cpp:100-113). */
 int wl_time_op(wl_ctx *ctx, int what, wl_field *out, const wl_field *in, double kappa, int sign,
                int warmup, int iters, double *ms_per_launch);

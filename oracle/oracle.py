@@ -55,11 +55,18 @@ def lib(variant=0):
                                   C.POINTER(C.c_double)]
         L.oracle_norm2.restype = C.c_double
         L.oracle_norm2.argtypes = [vp, dp, C.c_int]
+        L.oracle_set_threads.restype = C.c_int
+        L.oracle_set_threads.argtypes = [C.c_int]
         L.oracle_cg.restype = C.c_int
         L.oracle_cg.argtypes = [vp, dp, C.c_double, dp, dp, C.c_double, C.c_int,
                                 C.POINTER(C.c_double)]
         _libs[variant] = L
     return _libs[variant]
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops from now on; returns the previous count."""
+    return int(lib().oracle_set_threads(int(n)))
 
 
 def _p(a):

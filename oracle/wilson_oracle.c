@@ -37,6 +37,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define NDIM 4
 #define NDIRS 8
@@ -576,4 +579,17 @@ void oracle_hop_set_fused(const olat *l, const double *U, double kappa, const do
         for (int q = 0; q < 4; ++q)
             for (int c = 0; c < NC; ++c) o->c[q][c] = cscale(-kappa, acc.c[q][c]);
     }
+}
+
+/* Threads of the OpenMP loops from now on (n > 0), for single-core timings; returns the count
+ * before the call. */
+int oracle_set_threads(int n) {
+#ifdef _OPENMP
+    const int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
+#else
+    (void)n;
+    return 1;
+#endif
 }

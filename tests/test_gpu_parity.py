@@ -200,6 +200,27 @@ def test_argument_errors():
         W.Lattice((4, 4, 4, 5))
 
 
+def test_timed_copy_copies_and_checks_its_fields():
+    """wl_time_op what = 2 (the device-copy bandwidth the hopping term is compared with)."""
+    dims = (8, 8, 8, 8)
+    lat = W.Lattice(dims)
+    lat.upload_gauge(W.gen_gauge(dims, "unit"))
+    D = W.Dirac_Wilson_evenodd(0.12, lat)
+    a = W.Field(lat, EVEN, W.F64, 2)
+    for j in range(2):
+        a.upload(W.gen_spinor(dims, 40 + j, EVEN), rhs=j)
+    b = W.Field(lat, EVEN, W.F64, 2)
+    assert D.time(a, b, "copy", warmup=1, iters=3) > 0
+    assert a.download().tobytes() == b.download().tobytes()
+    with pytest.raises(W.WilsonError):   # rhs counts differ
+        D.time(a, W.Field(lat, EVEN), "copy")
+    other = W.Lattice(dims)
+    with pytest.raises(W.WilsonError):   # a field of another context
+        D.time(a, W.Field(other, EVEN, W.F64, 2), "copy")
+    other.close()
+    lat.close()
+
+
 @pytest.mark.parametrize("dims", [(32, 32, 32, 32), (24, 24, 24, 48), (16, 16, 16, 16)],
                          ids=lambda d: "x".join(map(str, d)))
 def test_traversal_and_workgroup_options_do_not_change_results(dims):

@@ -307,3 +307,22 @@ def test_oracle_reproduces_fixture(path):
 
 def test_fixtures_present():
     assert len(FIXTURES) >= 3
+
+
+def test_thread_count_does_not_change_the_oracle():
+    """The CPU baseline's single-core figure runs the same loops on one thread: per-site work,
+    identical bits."""
+    from oracle import oracle as O
+    dims = (4, 4, 4, 8)
+    orc = Oracle(dims)
+    U = W.gen_gauge(dims, "haar", 0, 7)
+    psi = W.gen_spinor(dims, 8, EVEN)
+    many = orc.hop_set(U, 0.12, psi, ODD, 1)
+    prev = O.set_threads(1)
+    try:
+        assert O.set_threads(0) == 1
+        one = orc.hop_set(U, 0.12, psi, ODD, 1)
+    finally:
+        O.set_threads(prev)
+    assert O.set_threads(0) == prev
+    assert one.tobytes() == many.tobytes()
