@@ -22,7 +22,8 @@ BUILD = os.path.join(ROOT, "hila_amd", "_build")
 LIBDIR = os.path.join(ROOT, "hila_amd", "lib")
 LIB = os.path.join(LIBDIR, "libhila_wilson.so")
 
-HIP_SOURCES = ["wl_rb_f16.hip", "wl_rb_f32.hip", "wl_rb_f64.hip", "wl_dslash.hip", "wl_multi.hip",
+HIP_SOURCES = ["wl_rb_f16.hip", "wl_rb_f32.hip", "wl_rb_f32r.hip", "wl_rb_f64.hip", "wl_rb_f64r.hip",
+               "wl_dslash.hip", "wl_multi.hip",
                "wl_blas.hip", "wl_api.hip", "wl_cg.hip", "wl_serve.hip", "wl_config.hip", "wl_mre.hip", "wl_force.hip"]
 # Per-source device flags.  wl_rb_f16.hip: without the v_dot2c_f32_f16 encoding (dot5-insts,
 # which accumulates in place and so needs a zero move to start every FP16 dot-product chain)

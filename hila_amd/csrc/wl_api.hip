@@ -112,12 +112,56 @@ RedArgs red_args(wl_ctx *c) {
     return ra;
 }
 
+// Stored entries per link of the links the hopping kernels of precision prec read: 6 for
+// two-row links (WL_OPT_LINK_ROWS = 2, FP64 and FP32), else 9.
+int link_format(const wl_ctx *c, int prec) { return c->link_rows == 2 && prec != WL_F16 ? 6 : 9; }
+
 const void *gauge_for(wl_ctx *c, int prec) {
+    if (link_format(c, prec) == 6) return prec == WL_F64 ? c->rows64 : c->rows32;
     return prec == WL_F64 ? c->gauge64 : (prec == WL_F32 ? c->gauge32 : c->gauge16);
+}
+
+// The largest |re|, |im| of U[2][j] - conj(row 0 x row 1)_j that two-row links accept: the
+// rebuilt third row then agrees with the stored one to rounding, and so do the operator's
+// results.
+constexpr double kRowsTol = 1e-13;
+
+// Checks the uploaded FP64 links for two-row storage (once per upload) and builds the two-row
+// copy of precision prec (FP64 or FP32).
+int ensure_rows(wl_ctx *c, int prec) {
+    if (c->rows_check == 0) {
+        const int nb = two_rows_blocks(c->g);
+        if (int rc = ensure_partials(c, (size_t)nb)) return rc;
+        HIPCHK(launch_two_rows(c->g, WL_F64, c->gauge64, nullptr, c->partials, c->stream));
+        std::vector<double> dev((size_t)nb);
+        HIPCHK(hipMemcpyAsync(dev.data(), c->partials, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double m = 0;
+        for (double d : dev) m = std::max(m, d);
+        c->rows_dev = m;
+        c->rows_check = m <= kRowsTol ? 1 : -1;
+    }
+    if (c->rows_check < 0)
+        return fail(WL_E_ARG,
+                    "two-row links (WL_OPT_LINK_ROWS = 2) need special unitary links: a stored third row differs "
+                    "from conj(row 0 x row 1) by %.3g (> %.0e)",
+                    c->rows_dev, kRowsTol);
+    void **rows = prec == WL_F64 ? &c->rows64 : &c->rows32;
+    bool *ok = prec == WL_F64 ? &c->rows64_ok : &c->rows32_ok;
+    if (!*rows) {
+        if (int rc = dev_alloc(c, rows, rows_elems(c->g) * (prec == WL_F64 ? 16 : 8), "two-row link copy")) return rc;
+    }
+    if (!*ok) {
+        HIPCHK(launch_two_rows(c->g, prec, c->gauge64, *rows, nullptr, c->stream));
+        *ok = true;
+    }
+    return WL_OK;
 }
 
 int ensure_gauge(wl_ctx *c, int prec) {
     if (!c->gauge_ok) return fail(WL_E_STATE, "no gauge field uploaded (wl_gauge_upload)");
+    if (link_format(c, prec) == 6) return ensure_rows(c, prec);
     if (prec == WL_F64) return WL_OK;
     if (prec == WL_F32) {
         if (!c->gauge32) {
@@ -150,6 +194,7 @@ int dslash_one(wl_ctx *c, int prec, void *out, int out_par, const void *in, cons
     L.in = in;
     L.xin = xin;
     L.gauge = gauge_for(c, prec);
+    L.lf = link_format(c, prec);
     L.a = a;
     L.b = 1.0 + c->op_shift;
     L.out_par = out_par;
@@ -272,6 +317,7 @@ int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, c
         L.in = in;
         L.xin = xin;
         L.gauge = gauge_for(c, prec);
+        L.lf = link_format(c, prec);
         L.a = a;
         L.b = 1.0 + c->op_shift;
         L.out_par = out_par;
@@ -336,6 +382,7 @@ int gauge_upload_async(wl_ctx *c, const double *const U[4]) {
     c->gauge_ok = true;
     c->gauge32_ok = false;
     c->gauge16_ok = false;
+    links_changed(c);
     return WL_OK;
 }
 
@@ -464,8 +511,9 @@ extern "C" int wl_destroy(wl_ctx *c) {
         f->ctx = nullptr;
     }
     c->user_fields.clear();
-    for (void *p : {(void *)c->st_mr, (void *)c->flags_d, (void *)c->gauge64, c->gauge32, c->gauge16,
-                    c->staging, (void *)c->partials, (void *)c->st, (void *)c->scratch_d, c->force64, c->mom64})
+    for (void *p : {(void *)c->st_mr, (void *)c->flags_d, (void *)c->gauge64, c->gauge32, c->gauge16, c->rows64,
+                    c->rows32, c->staging, (void *)c->partials, (void *)c->st, (void *)c->scratch_d, c->force64,
+                    c->mom64})
         if (p) (void)hipFree(p);
     if (c->st_host) (void)hipHostFree(c->st_host);
     if (c->scratch_h) (void)hipHostFree(c->scratch_h);
@@ -735,6 +783,19 @@ extern "C" int wl_set_option(wl_ctx *c, int option, int value) {
         if (value != 0 && value != 1) return fail(WL_E_ARG, "pipeline option must be 0 or 1");
         c->pipeline = value;
         return WL_OK;
+    case WL_OPT_LINK_ROWS: {
+        if (value != 2 && value != 3) return fail(WL_E_ARG, "link rows must be 3 (full links) or 2 (two-row links)");
+        const int prev = c->link_rows;
+        c->link_rows = value;
+        if (value == 2 && c->gauge_ok) {  // check the links now, not at the next operator call
+            DeviceGuard dg(c->device);
+            if (int rc = ensure_rows(c, WL_F64)) {
+                c->link_rows = prev;
+                return rc;
+            }
+        }
+        return WL_OK;
+    }
     case WL_OPT_PLANE_PAD: {
         if (value != -1 && (value < 0 || value > 65536 || value % 64 != 0))
             return fail(WL_E_ARG, "plane padding must be -1 or a multiple of 64 in 0..65536");
@@ -761,6 +822,7 @@ extern "C" int wl_get_option(const wl_ctx *c, int option, int *value) {
     case WL_OPT_LDS_RESERVE: *value = lds_reserve_of(c); return WL_OK;
     case WL_OPT_GRAPHS: *value = c->use_graphs ? 1 : 0; return WL_OK;
     case WL_OPT_PIPELINE: *value = c->pipeline; return WL_OK;
+    case WL_OPT_LINK_ROWS: *value = c->link_rows; return WL_OK;
     default: return fail(WL_E_ARG, "unknown option %d", option);
     }
 }

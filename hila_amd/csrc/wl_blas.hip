@@ -89,6 +89,50 @@ __global__ __launch_bounds__(kBlock) void k_convert_h2(const cx<double> *__restr
     out[blk * 9 * S + link_entry<f16s>(e, S, k)] = q;
 }
 
+// Two-row link copy for the hopping kernels (rows 0 and 1 of every link, in the 6S-element
+// blocks of ld_link_raw's LF = 6 layout) from the FP64 planes; out may be null (check only).
+// With dev non-null each workgroup also writes the largest |re|, |im| of
+// U[2][j] - conj(row 0 x row 1)_j over its links to dev[workgroup]: how far the stored third
+// row is from the one the two-row kernels rebuild.  Thread i: link block i / vh (parity,
+// direction), site i % vh.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_two_rows(const cx<double> *__restrict__ in, cx<R> *__restrict__ out,
+                                                     int vh, int S, double *dev) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    double d = 0;
+    if (i < (size_t)8 * vh) {
+        const int blk = (int)(i / vh), k = (int)(i - (size_t)blk * vh);
+        const cx<double> *src = in + (size_t)blk * 9 * S + k;
+        cx<double> u[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) u[e] = src[(size_t)e * S];
+        if (out) {
+#pragma unroll
+            for (int e = 0; e < 6; ++e)
+                out[(size_t)blk * 6 * S + link_entry<R>(e, S, k)] = {(R)u[e].x, (R)u[e].y};
+        }
+        if (dev) {
+            cx<double> w[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) w[e] = u[e];
+            third_row(w);
+#pragma unroll
+            for (int e = 6; e < 9; ++e) d = fmax(d, fmax(fabs(w[e].x - u[e].x), fabs(w[e].y - u[e].y)));
+            if (d != d) d = INFINITY;  // NaN entries fail the check
+        }
+    }
+    if (dev) {  // grid-uniform: every thread reaches the barriers
+        __shared__ double part[kBlock];
+        part[threadIdx.x] = d;
+        __syncthreads();
+        for (int o = kBlock / 2; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + o]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) dev[blockIdx.x] = part[0];
+    }
+}
+
 // ------------------------------------------------------------------ reductions ---------
 
 template <typename T>
@@ -264,6 +308,21 @@ hipError_t launch_gauge_convert(const Geom &g, int prec, const void *gauge64, vo
     else if (prec == WL_F16)
         hipLaunchKernelGGL(k_convert_h2, grid, dim3(kBlock), 0, s, (const cx<double> *)gauge64,
                            (h2 *)gauge_lo, n, g.stride);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+int two_rows_blocks(const Geom &g) { return (int)(((size_t)8 * g.vh + kBlock - 1) / kBlock); }
+
+hipError_t launch_two_rows(const Geom &g, int prec, const void *gauge64, void *rows, double *dev, hipStream_t s) {
+    const dim3 grid((unsigned)two_rows_blocks(g));
+    if (prec == WL_F64)
+        hipLaunchKernelGGL(k_two_rows<double>, grid, dim3(kBlock), 0, s, (const cx<double> *)gauge64,
+                           (cx<double> *)rows, g.vh, g.stride, dev);
+    else if (prec == WL_F32)
+        hipLaunchKernelGGL(k_two_rows<float>, grid, dim3(kBlock), 0, s, (const cx<double> *)gauge64,
+                           (cx<float> *)rows, g.vh, g.stride, dev);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

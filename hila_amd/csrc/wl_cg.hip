@@ -201,6 +201,7 @@ int cg_mixed(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, i
             L.in = t;
             L.xin = Dp;
             L.gauge = gauge_for(c, prec);
+            L.lf = link_format(c, prec);
             L.a = kappa;
             L.b = 1.0 + c->op_shift;
             L.out_par = 0;

@@ -211,6 +211,7 @@ int config_upload(wl_ctx *c, const char *path) {
     if (int rc = pp.init(std::min(kChunk, dir_bytes))) return rc;
     // From here on a failure leaves the gauge partly overwritten: mark it unusable.
     c->gauge_ok = c->gauge32_ok = c->gauge16_ok = false;
+    links_changed(c);
     size_t n_chunk = 0;
     for (int mu = 0; mu < 4; ++mu) {
         for (size_t off = 0; off < dir_bytes; off += pp.bytes, ++n_chunk) {

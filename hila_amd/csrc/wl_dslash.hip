@@ -43,7 +43,8 @@ struct DslashArgs {
     int fin;
 };
 
-template <typename R, int SIGN, int EPI>
+// LF: stored entries per link, 9 or 6 (two-row links, wl_store.h).
+template <typename R, int SIGN, int EPI, int LF>
 __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
     if (A.done && *A.done) return;
     const int k = lane_site(A.g);
@@ -51,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
     double nrm = 0;
     if (k < A.g.vh) {
         cx<R> acc[4][3];
-        hop_sum<R, SIGN>(acc, A.g, k, A.out_par, A.in, A.gauge, 1);
+        hop_sum<R, SIGN, true, LF>(acc, A.g, k, A.out_par, A.in, A.gauge, 1);
 
         const R a = A.a;
         if constexpr (EPI == EPI_SET) {
@@ -129,7 +130,10 @@ static hipError_t launch_t(const Geom &g, const DslashLaunch &L, hipStream_t s) 
     A.fin = L.fin;
     const int blk = g.blk;
     const int nb = (g.vh + blk - 1) / blk;
-    hipLaunchKernelGGL((k_dslash<R, SIGN, EPI>), dim3(nb), dim3(blk), g.lds, s, A);
+    if (L.lf == 6)
+        hipLaunchKernelGGL((k_dslash<R, SIGN, EPI, 6>), dim3(nb), dim3(blk), g.lds, s, A);
+    else
+        hipLaunchKernelGGL((k_dslash<R, SIGN, EPI, 9>), dim3(nb), dim3(blk), g.lds, s, A);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || (EPI != EPI_AXPY_NORM && EPI != EPI_CGUPD)) return e;
     return launch_finish(L.red, nb, L.fin, L.done, s);

@@ -24,13 +24,13 @@ struct HopBuf {
 
 // The neighbour spinor at site kn of the field `in` and the link entries at site kl of the link
 // block `link`, in the packed layouts of wl_store.h (FP64 and FP32 storage).
-template <bool NTL = true, typename R>
+template <bool NTL = true, int LF = 9, typename R>
 __device__ __forceinline__ void hop_load(HopBuf<R> &b, const cx<R> *__restrict__ in,
                                          const cx<R> *__restrict__ link, int Ssp, int kn,
                                          int Slk, int kl) {
     cx<R> w[12], u[9];
     ld_spinor_raw<false, R>(w, in, Ssp, kn);
-    ld_link_raw<NTL, R>(u, link, Slk, kl);
+    ld_link_raw<NTL, R, LF>(u, link, Slk, kl);
 #pragma unroll
     for (int c = 0; c < 12; ++c) b.w[c / 3][c % 3] = w[c];
 #pragma unroll
@@ -65,8 +65,8 @@ __device__ __forceinline__ void hop_compute(cx<R> (&acc)[4][3], const HopBuf<R> 
 // Hop H = 2 mu + (0 forward, 1 backward): issue hop H+1's loads, then compute hop H.
 // nsp: neighbour element offsets of the spinor field (site index times the rhs count);
 // nl: neighbour site indices (backward links live at the neighbour); S: link plane stride;
-// Ssp: spinor plane stride.
-template <typename R, int SIGN, int H, bool NTL = true>
+// Ssp: spinor plane stride; LF: stored entries per link (9, or 6 for two-row links).
+template <typename R, int SIGN, int H, bool NTL = true, int LF = 9>
 __device__ __forceinline__ void hop_chain(cx<R> (&acc)[4][3], HopBuf<R> (&buf)[2],
                                           const cx<R> *__restrict__ in, const cx<R> *gout,
                                           const cx<R> *gin, int Ssp, int S, int k,
@@ -76,18 +76,19 @@ __device__ __forceinline__ void hop_chain(cx<R> (&acc)[4][3], HopBuf<R> (&buf)[2
         if constexpr (HN < 8) {
             constexpr int MN = HN / 2;
             if constexpr (HN % 2 == 0)
-                hop_load<NTL>(buf[HN & 1], in, gout + MN * 9 * S, Ssp, nsp.fwd[MN], S, k);
+                hop_load<NTL, LF>(buf[HN & 1], in, gout + MN * LF * S, Ssp, nsp.fwd[MN], S, k);
             else
-                hop_load<NTL>(buf[HN & 1], in, gin + MN * 9 * S, Ssp, nsp.bwd[MN], S, nl.bwd[MN]);
+                hop_load<NTL, LF>(buf[HN & 1], in, gin + MN * LF * S, Ssp, nsp.bwd[MN], S, nl.bwd[MN]);
         }
         hop_compute<H / 2, SIGN, (H % 2 == 0)>(acc, buf[H & 1], (H % 2 == 0) ? sf : sb);
-        hop_chain<R, SIGN, H + 1, NTL>(acc, buf, in, gout, gin, Ssp, S, k, nsp, nl, sf, sb);
+        hop_chain<R, SIGN, H + 1, NTL, LF>(acc, buf, in, gout, gin, Ssp, S, k, nsp, nl, sf, sb);
     }
 }
 
 // acc = H_SIGN in at output site k (parity p): the whole pipelined 8-hop sum.  NTL: stream the
-// links past the caches (single rhs, where each link is read once per application).
-template <typename R, int SIGN, bool NTL = true>
+// links past the caches (single rhs, where each link is read once per application).  LF:
+// stored entries per link, as hop_chain.
+template <typename R, int SIGN, bool NTL = true, int LF = 9>
 __device__ __forceinline__ void hop_sum(cx<R> (&acc)[4][3], const Geom &g, int k, int p,
                                         const cx<R> *__restrict__ in, const cx<R> *gauge, int N) {
     const int S = g.stride;
@@ -100,8 +101,8 @@ __device__ __forceinline__ void hop_sum(cx<R> (&acc)[4][3], const Geom &g, int k
             nsp.bwd[m] = n.bwd[m] * N;
         }
     }
-    const cx<R> *gout = gauge + (size_t)p * 36 * S;
-    const cx<R> *gin = gauge + (size_t)(1 - p) * 36 * S;
+    const cx<R> *gout = gauge + (size_t)p * 4 * LF * S;
+    const cx<R> *gin = gauge + (size_t)(1 - p) * 4 * LF * S;
     const R sf = (g.antiperiodic && n.fwd_edge_t) ? R(-1) : R(1);
     const R sb = (g.antiperiodic && n.bwd_edge_t) ? R(-1) : R(1);
 #pragma unroll
@@ -109,8 +110,8 @@ __device__ __forceinline__ void hop_sum(cx<R> (&acc)[4][3], const Geom &g, int k
 #pragma unroll
         for (int c = 0; c < 3; ++c) acc[s][c] = {R(0), R(0)};
     HopBuf<R> buf[2];
-    hop_load<NTL>(buf[0], in, gout, S * N, nsp.fwd[0], S, k);
-    hop_chain<R, SIGN, 0, NTL>(acc, buf, in, gout, gin, S * N, S, k, nsp, n, sf, sb);
+    hop_load<NTL, LF>(buf[0], in, gout, S * N, nsp.fwd[0], S, k);
+    hop_chain<R, SIGN, 0, NTL, LF>(acc, buf, in, gout, gin, S * N, S, k, nsp, n, sf, sb);
 }
 
 

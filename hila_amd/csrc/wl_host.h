@@ -55,6 +55,12 @@ void field_free(wl_field *f);
 int ensure_tmp(wl_ctx *c, int prec, wl_field **slot, int parity);
 RedArgs red_args(wl_ctx *c);
 const void *gauge_for(wl_ctx *c, int prec);
+int link_format(const wl_ctx *c, int prec);
+// the links were overwritten: two-row copies and their check are stale
+inline void links_changed(wl_ctx *c) {
+    c->rows64_ok = c->rows32_ok = false;
+    c->rows_check = 0;
+}
 int ensure_gauge(wl_ctx *c, int prec);
 int check_sign(int sign);
 int ensure_partials(wl_ctx *c, size_t n);

@@ -40,6 +40,7 @@ struct DslashLaunch {
     const void *gauge;
     double a;
     double b = 1.0;  // coefficient of xin in the epilogues that read it: out = b xin + a H in
+    int lf = 9;      // stored entries per link of `gauge`: 9, or 6 (two-row links)
     int out_par;   // 0 even, 1 odd
     int sign;      // +1 apply, -1 dagger
     int epi;
@@ -62,6 +63,11 @@ hipError_t launch_soa_to_aos(int prec, const Geom &g, const void *planes, double
                              int j, hipStream_t s);
 hipError_t launch_gauge_to_soa(const Geom &g, int mu, const double *aos_full, void *gauge64,
                                hipStream_t s);
+// two-row link copy (prec F64 or F32; rows may be null) and, with dev, the per-workgroup
+// largest deviation of the stored third row from conj(row 0 x row 1) (two_rows_blocks values)
+hipError_t launch_two_rows(const Geom &g, int prec, const void *gauge64, void *rows, double *dev, hipStream_t s);
+int two_rows_blocks(const Geom &g);
+inline size_t rows_elems(const Geom &g) { return (size_t)2 * 4 * 6 * g.stride; }
 hipError_t launch_gauge_convert(const Geom &g, int prec, const void *gauge64, void *gauge_lo,
                                 hipStream_t s);
 // direction mu of link planes (gauge layout, FP64) -> 18 doubles per site, sites in
@@ -104,6 +110,7 @@ struct MrLaunch {
     const void *gauge;
     double a;
     double b = 1.0;           // coefficient of xin (see DslashLaunch)
+    int lf = 9;               // stored entries per link (see DslashLaunch)
     int out_par, sign, epi;
     int j0, NB, NR;
     bool link_nt;             // nontemporal link loads
@@ -177,6 +184,15 @@ struct wl_ctx {
     bool gauge_ok = false;
     bool gauge32_ok = false;
     bool gauge16_ok = false;
+    // two-row link copies (rows 0 and 1 of every link) for the FP64 and FP32 hopping kernels,
+    // used when link_rows == 2 and the links passed the special-unitarity check
+    void *rows64 = nullptr;
+    void *rows32 = nullptr;
+    bool rows64_ok = false;
+    bool rows32_ok = false;
+    int rows_check = 0;                // 0 not checked since the upload, 1 passed, -1 failed
+    double rows_dev = 0;               // largest |U[2][j] - conj(row 0 x row 1)_j| found
+    int link_rows = 3;                 // WL_OPT_LINK_ROWS
     void *staging = nullptr;
     size_t staging_bytes = 0;
     double *partials = nullptr;

@@ -216,10 +216,13 @@ int mr_segments(const Geom &g) { return (g.vh + 63) / 64; }
 hipError_t launch_dslash_mr_f64(const Geom &g, const MrLaunch &L, hipStream_t s);
 hipError_t launch_dslash_mr_f32(const Geom &g, const MrLaunch &L, hipStream_t s);
 hipError_t launch_dslash_mr_f16(const Geom &g, const MrLaunch &L, hipStream_t s);
+hipError_t launch_dslash_mr_f64_rows(const Geom &g, const MrLaunch &L, hipStream_t s);
+hipError_t launch_dslash_mr_f32_rows(const Geom &g, const MrLaunch &L, hipStream_t s);
 
 hipError_t launch_dslash_mr(int prec, const Geom &g, const MrLaunch &L, hipStream_t s) {
-    if (prec == WL_F64) return launch_dslash_mr_f64(g, L, s);
-    if (prec == WL_F32) return launch_dslash_mr_f32(g, L, s);
+    if (L.lf != 9 && (L.lf != 6 || prec == WL_F16)) return hipErrorInvalidValue;
+    if (prec == WL_F64) return L.lf == 6 ? launch_dslash_mr_f64_rows(g, L, s) : launch_dslash_mr_f64(g, L, s);
+    if (prec == WL_F32) return L.lf == 6 ? launch_dslash_mr_f32_rows(g, L, s) : launch_dslash_mr_f32(g, L, s);
     if (prec == WL_F16) return launch_dslash_mr_f16(g, L, s);
     return hipErrorInvalidValue;
 }

@@ -77,14 +77,14 @@ __device__ __forceinline__ void hop_pk(cx<float> (&acc)[4][3], const h2 (&w)[12]
 // does not displace the neighbour spinors from the caches.  For the scaled format the site
 // scale of each neighbour is applied to the product U h (6 complex numbers) instead of to the
 // 12 components of the spinor: the hop is linear.
-template <int MU, int SIGN, bool FWD, int NR, bool NTL, typename T>
+template <int MU, int SIGN, bool FWD, int NR, bool NTL, typename T, int LF>
 __device__ __forceinline__ void rb_hop(cx<typename St<T>::R> (&acc)[NR][4][3], const bool (&act)[NR],
                                        const typename St<T>::E *__restrict__ in, size_t rs,
                                        const typename St<T>::E *__restrict__ link, int S, int kn,
                                        int kl, typename St<T>::R sg) {
     using R = typename St<T>::R;
     typename St<T>::E ur[9];
-    ld_link_raw<NTL, T>(ur, link, S, kl);
+    ld_link_raw<NTL, T, LF>(ur, link, S, kl);
     cx<R> u[3][3];
     if constexpr (!St<T>::scaled) {
 #pragma unroll
@@ -140,12 +140,12 @@ constexpr bool rb_pipe_ok() {
     return sizeof(typename St<T>::R) == 4 && NR <= 2;  // FP16 with 4 per lane spills
 }
 
-template <int NR, bool NTL, typename T>
+template <int NR, bool NTL, typename T, int LF>
 __device__ __forceinline__ void rb_load(RbBuf<T, NR> &b, const bool (&act)[NR],
                                         const typename St<T>::E *__restrict__ in, size_t rs,
                                         const typename St<T>::E *__restrict__ link, int S, int kn,
                                         int kl) {
-    ld_link_raw<NTL, T>(b.u, link, S, kl);
+    ld_link_raw<NTL, T, LF>(b.u, link, S, kl);
 #pragma unroll
     for (int r = 0; r < NR; ++r)
         if (act[r]) {
@@ -197,7 +197,7 @@ __device__ __forceinline__ void rb_compute(cx<typename St<T>::R> (&acc)[NR][4][3
 }
 
 // Hop H = 2 mu + (0 forward, 1 backward): issue hop H+1's loads, then compute hop H.
-template <typename T, int SIGN, int H, int NR, bool NTL>
+template <typename T, int SIGN, int H, int NR, bool NTL, int LF>
 __device__ __forceinline__ void rb_chain(cx<typename St<T>::R> (&acc)[NR][4][3], RbBuf<T, NR> (&buf)[2],
                                          const bool (&act)[NR], const typename St<T>::E *__restrict__ in,
                                          size_t rs, const typename St<T>::E *gout,
@@ -208,12 +208,12 @@ __device__ __forceinline__ void rb_chain(cx<typename St<T>::R> (&acc)[NR][4][3],
         if constexpr (HN < 8) {
             constexpr int MN = HN / 2;
             if constexpr (HN % 2 == 0)
-                rb_load<NR, NTL, T>(buf[HN & 1], act, in, rs, gout + MN * 9 * S, S, n.fwd[MN], k);
+                rb_load<NR, NTL, T, LF>(buf[HN & 1], act, in, rs, gout + MN * LF * S, S, n.fwd[MN], k);
             else
-                rb_load<NR, NTL, T>(buf[HN & 1], act, in, rs, gin + MN * 9 * S, S, n.bwd[MN], n.bwd[MN]);
+                rb_load<NR, NTL, T, LF>(buf[HN & 1], act, in, rs, gin + MN * LF * S, S, n.bwd[MN], n.bwd[MN]);
         }
         rb_compute<H / 2, SIGN, (H % 2 == 0), NR, T>(acc, buf[H & 1], act, (H % 2 == 0) ? sf : sb);
-        rb_chain<T, SIGN, H + 1, NR, NTL>(acc, buf, act, in, rs, gout, gin, S, k, n, sf, sb);
+        rb_chain<T, SIGN, H + 1, NR, NTL, LF>(acc, buf, act, in, rs, gout, gin, S, k, n, sf, sb);
     }
 }
 
@@ -292,7 +292,7 @@ __device__ __forceinline__ void rb_partials(const RbArgs<T> &A, const double (&n
     }
 }
 
-template <typename T, int SIGN, int EPI, int NR, bool NTL, bool PIPE>
+template <typename T, int SIGN, int EPI, int NR, bool NTL, bool PIPE, int LF>
 __global__ __launch_bounds__(kBlock, 2) void k_dslash_rb(RbArgs<T> A) {
     using R = typename St<T>::R;
     using E = typename St<T>::E;
@@ -314,8 +314,8 @@ __global__ __launch_bounds__(kBlock, 2) void k_dslash_rb(RbArgs<T> A) {
     for (int r = 0; r < NR; ++r) nrm[r] = 0;
     if (k < A.g.vh) {
         const SiteNb n = neighbours(A.g, k, A.out_par);
-        const E *gout = A.gauge + (size_t)A.out_par * 36 * S;
-        const E *gin = A.gauge + (size_t)(1 - A.out_par) * 36 * S;
+        const E *gout = A.gauge + (size_t)A.out_par * 4 * LF * S;
+        const E *gin = A.gauge + (size_t)(1 - A.out_par) * 4 * LF * S;
         const R sf = (A.g.antiperiodic && n.fwd_edge_t) ? R(-1) : R(1);
         const R sb = (A.g.antiperiodic && n.bwd_edge_t) ? R(-1) : R(1);
         const E *in = A.in + jf * rs;
@@ -328,17 +328,17 @@ __global__ __launch_bounds__(kBlock, 2) void k_dslash_rb(RbArgs<T> A) {
                 for (int c = 0; c < 3; ++c) acc[r][s][c] = {R(0), R(0)};
         if constexpr (PIPE) {
             RbBuf<T, NR> buf[2];
-            rb_load<NR, NTL, T>(buf[0], act, in, rs, gout, S, n.fwd[0], k);
-            rb_chain<T, SIGN, 0, NR, NTL>(acc, buf, act, in, rs, gout, gin, S, k, n, sf, sb);
+            rb_load<NR, NTL, T, LF>(buf[0], act, in, rs, gout, S, n.fwd[0], k);
+            rb_chain<T, SIGN, 0, NR, NTL, LF>(acc, buf, act, in, rs, gout, gin, S, k, n, sf, sb);
         } else {
-            rb_hop<0, SIGN, true, NR, NTL, T>(acc, act, in, rs, gout, S, n.fwd[0], k, R(1));
-            rb_hop<0, SIGN, false, NR, NTL, T>(acc, act, in, rs, gin, S, n.bwd[0], n.bwd[0], R(1));
-            rb_hop<1, SIGN, true, NR, NTL, T>(acc, act, in, rs, gout + 9 * S, S, n.fwd[1], k, R(1));
-            rb_hop<1, SIGN, false, NR, NTL, T>(acc, act, in, rs, gin + 9 * S, S, n.bwd[1], n.bwd[1], R(1));
-            rb_hop<2, SIGN, true, NR, NTL, T>(acc, act, in, rs, gout + 18 * S, S, n.fwd[2], k, R(1));
-            rb_hop<2, SIGN, false, NR, NTL, T>(acc, act, in, rs, gin + 18 * S, S, n.bwd[2], n.bwd[2], R(1));
-            rb_hop<3, SIGN, true, NR, NTL, T>(acc, act, in, rs, gout + 27 * S, S, n.fwd[3], k, sf);
-            rb_hop<3, SIGN, false, NR, NTL, T>(acc, act, in, rs, gin + 27 * S, S, n.bwd[3], n.bwd[3], sb);
+            rb_hop<0, SIGN, true, NR, NTL, T, LF>(acc, act, in, rs, gout, S, n.fwd[0], k, R(1));
+            rb_hop<0, SIGN, false, NR, NTL, T, LF>(acc, act, in, rs, gin, S, n.bwd[0], n.bwd[0], R(1));
+            rb_hop<1, SIGN, true, NR, NTL, T, LF>(acc, act, in, rs, gout + LF * S, S, n.fwd[1], k, R(1));
+            rb_hop<1, SIGN, false, NR, NTL, T, LF>(acc, act, in, rs, gin + LF * S, S, n.bwd[1], n.bwd[1], R(1));
+            rb_hop<2, SIGN, true, NR, NTL, T, LF>(acc, act, in, rs, gout + 2 * LF * S, S, n.fwd[2], k, R(1));
+            rb_hop<2, SIGN, false, NR, NTL, T, LF>(acc, act, in, rs, gin + 2 * LF * S, S, n.bwd[2], n.bwd[2], R(1));
+            rb_hop<3, SIGN, true, NR, NTL, T, LF>(acc, act, in, rs, gout + 3 * LF * S, S, n.fwd[3], k, sf);
+            rb_hop<3, SIGN, false, NR, NTL, T, LF>(acc, act, in, rs, gin + 3 * LF * S, S, n.bwd[3], n.bwd[3], sb);
         }
 
         rb_epilogue<T, EPI, NR>(A, acc, act, jf, k, nrm);
@@ -395,7 +395,7 @@ __device__ __forceinline__ void rb_chain_lds(cx<typename St<T>::R> (&acc)[NR][4]
     }
 }
 
-template <typename T, int SIGN, int EPI, int NR, bool PIPE>
+template <typename T, int SIGN, int EPI, int NR, bool PIPE, int LF>
 __global__ __launch_bounds__(512, 1) void k_dslash_rb_lds(RbArgs<T> A) {
     using R = typename St<T>::R;
     using E = typename St<T>::E;
@@ -420,14 +420,25 @@ __global__ __launch_bounds__(512, 1) void k_dslash_rb_lds(RbArgs<T> A) {
     const bool live = k < A.g.vh;
     SiteNb n{};
     if (live) n = neighbours(A.g, k, A.out_par);
-    const E *gout = A.gauge + (size_t)A.out_par * 36 * S;
-    const E *gin = A.gauge + (size_t)(1 - A.out_par) * 36 * S;
+    const E *gout = A.gauge + (size_t)A.out_par * 4 * LF * S;
+    const E *gin = A.gauge + (size_t)(1 - A.out_par) * 4 * LF * S;
     if (live) {
-        for (int idx = chunk; idx < 72; idx += nw) {
-            const int h = idx / 9, e = idx - 9 * h, mu = h >> 1;
-            const E *src = (h & 1) ? gin + (size_t)mu * 9 * S + link_entry<T>(e, S, n.bwd[mu])
-                                   : gout + (size_t)mu * 9 * S + link_entry<T>(e, S, k);
-            lu[idx * 64 + lane] = ldraw<true>(src);
+        if constexpr (LF == 9) {
+            for (int idx = chunk; idx < 72; idx += nw) {
+                const int h = idx / 9, e = idx - 9 * h, mu = h >> 1;
+                const E *src = (h & 1) ? gin + (size_t)mu * 9 * S + link_entry<T>(e, S, n.bwd[mu])
+                                       : gout + (size_t)mu * 9 * S + link_entry<T>(e, S, k);
+                lu[idx * 64 + lane] = ldraw<true>(src);
+            }
+        } else {  // two-row links: a wave loads whole links and stages the rebuilt 9 entries
+            for (int h = chunk; h < 8; h += nw) {
+                const int mu = h >> 1;
+                E u[9];
+                ld_link_raw<true, T, LF>(u, ((h & 1) ? gin : gout) + (size_t)mu * LF * S, S,
+                                         (h & 1) ? n.bwd[mu] : k);
+#pragma unroll
+                for (int e = 0; e < 9; ++e) lu[(h * 9 + e) * 64 + lane] = u[e];
+            }
         }
     }
     __syncthreads();
@@ -450,7 +461,7 @@ __global__ __launch_bounds__(512, 1) void k_dslash_rb_lds(RbArgs<T> A) {
     rb_partials<T, EPI, NR>(A, nrm, jf, o);
 }
 
-template <typename T, int SIGN, int EPI, int NR, bool NTL, bool PIPE>
+template <typename T, int SIGN, int EPI, int NR, bool NTL, bool PIPE, int LF>
 inline hipError_t rb_launch(const Geom &g, const MrLaunch &L, hipStream_t s) {
     using E = typename St<T>::E;
     using R = typename St<T>::R;
@@ -475,16 +486,16 @@ inline hipError_t rb_launch(const Geom &g, const MrLaunch &L, hipStream_t s) {
     A.pstride = mr_segments(g);
     const long waves = (long)A.pstride * A.C;
     const int wpb = g.blk / 64;
-    hipLaunchKernelGGL((k_dslash_rb<T, SIGN, EPI, NR, NTL, PIPE>), dim3((unsigned)((waves + wpb - 1) / wpb)),
+    hipLaunchKernelGGL((k_dslash_rb<T, SIGN, EPI, NR, NTL, PIPE, LF>), dim3((unsigned)((waves + wpb - 1) / wpb)),
                        dim3(g.blk), g.lds, s, A);
     return hipGetLastError();
 }
 
-template <typename T, int SIGN, int EPI, int NR, bool NTL>
+template <typename T, int SIGN, int EPI, int NR, bool NTL, int LF>
 inline hipError_t rb_pipe(const Geom &g, const MrLaunch &L, hipStream_t s) {
     if constexpr (rb_pipe_ok<T, NR>())
-        if (L.pipe) return rb_launch<T, SIGN, EPI, NR, NTL, true>(g, L, s);
-    return rb_launch<T, SIGN, EPI, NR, NTL, false>(g, L, s);
+        if (L.pipe) return rb_launch<T, SIGN, EPI, NR, NTL, true, LF>(g, L, s);
+    return rb_launch<T, SIGN, EPI, NR, NTL, false, LF>(g, L, s);
 }
 
 // Spinor pipelining in the LDS-staged kernel (no link registers): FP32 arithmetic up to 4 per
@@ -494,7 +505,7 @@ constexpr bool rb_lds_pipe_ok() {
     return sizeof(typename St<T>::R) == 4 ? NR <= 4 : NR <= 1;
 }
 
-template <typename T, int SIGN, int EPI, int NR, bool PIPE>
+template <typename T, int SIGN, int EPI, int NR, bool PIPE, int LF>
 inline hipError_t rb_launch_lds(const Geom &g, const MrLaunch &L, hipStream_t s) {
     using E = typename St<T>::E;
     using R = typename St<T>::R;
@@ -518,43 +529,43 @@ inline hipError_t rb_launch_lds(const Geom &g, const MrLaunch &L, hipStream_t s)
     A.partials = L.partials;
     A.pstride = mr_segments(g);
     const size_t lds = (size_t)72 * 64 * sizeof(E) + (size_t)g.lds;
-    hipLaunchKernelGGL((k_dslash_rb_lds<T, SIGN, EPI, NR, PIPE>), dim3((unsigned)A.pstride), dim3(64 * A.C),
+    hipLaunchKernelGGL((k_dslash_rb_lds<T, SIGN, EPI, NR, PIPE, LF>), dim3((unsigned)A.pstride), dim3(64 * A.C),
                        lds, s, A);
     return hipGetLastError();
 }
 
-template <typename T, int SIGN, int EPI, int NR>
+template <typename T, int SIGN, int EPI, int NR, int LF>
 inline hipError_t rb_ntl(const Geom &g, const MrLaunch &L, hipStream_t s) {
     if (L.lds_links && (L.NB + NR - 1) / NR <= 8) {
         if constexpr (rb_lds_pipe_ok<T, NR>())
-            if (L.pipe) return rb_launch_lds<T, SIGN, EPI, NR, true>(g, L, s);
-        return rb_launch_lds<T, SIGN, EPI, NR, false>(g, L, s);
+            if (L.pipe) return rb_launch_lds<T, SIGN, EPI, NR, true, LF>(g, L, s);
+        return rb_launch_lds<T, SIGN, EPI, NR, false, LF>(g, L, s);
     }
-    return L.link_nt ? rb_pipe<T, SIGN, EPI, NR, true>(g, L, s) : rb_pipe<T, SIGN, EPI, NR, false>(g, L, s);
+    return L.link_nt ? rb_pipe<T, SIGN, EPI, NR, true, LF>(g, L, s) : rb_pipe<T, SIGN, EPI, NR, false, LF>(g, L, s);
 }
 
-template <typename T, int SIGN, int NR>
+template <typename T, int SIGN, int NR, int LF>
 inline hipError_t rb_epi(const Geom &g, const MrLaunch &L, hipStream_t s) {
     switch (L.epi) {
-    case EPI_SET: return rb_ntl<T, SIGN, EPI_SET, NR>(g, L, s);
-    case EPI_AXPY: return rb_ntl<T, SIGN, EPI_AXPY, NR>(g, L, s);
-    case EPI_AXPY_NORM: return rb_ntl<T, SIGN, EPI_AXPY_NORM, NR>(g, L, s);
-    case EPI_CGUPD: return rb_ntl<T, SIGN, EPI_CGUPD, NR>(g, L, s);
+    case EPI_SET: return rb_ntl<T, SIGN, EPI_SET, NR, LF>(g, L, s);
+    case EPI_AXPY: return rb_ntl<T, SIGN, EPI_AXPY, NR, LF>(g, L, s);
+    case EPI_AXPY_NORM: return rb_ntl<T, SIGN, EPI_AXPY_NORM, NR, LF>(g, L, s);
+    case EPI_CGUPD: return rb_ntl<T, SIGN, EPI_CGUPD, NR, LF>(g, L, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 // FP32 arithmetic takes up to 4 right-hand sides per lane; FP64 up to 2 (3 or 4 spill
-// registers).
-template <typename T, int SIGN>
+// registers).  LF: stored entries per link (9, or 6 for two-row links).
+template <typename T, int SIGN, int LF = 9>
 inline hipError_t rb_nr(const Geom &g, const MrLaunch &L, hipStream_t s) {
     switch (L.NR) {
-    case 1: return rb_epi<T, SIGN, 1>(g, L, s);
-    case 2: return rb_epi<T, SIGN, 2>(g, L, s);
+    case 1: return rb_epi<T, SIGN, 1, LF>(g, L, s);
+    case 2: return rb_epi<T, SIGN, 2, LF>(g, L, s);
     default:
         if constexpr (sizeof(typename St<T>::R) == 4) {
-            if (L.NR == 3) return rb_epi<T, SIGN, 3>(g, L, s);
-            if (L.NR == 4) return rb_epi<T, SIGN, 4>(g, L, s);
+            if (L.NR == 3) return rb_epi<T, SIGN, 3, LF>(g, L, s);
+            if (L.NR == 4) return rb_epi<T, SIGN, 4, LF>(g, L, s);
         }
         return hipErrorInvalidValue;
     }

@@ -148,17 +148,37 @@ __device__ __forceinline__ void ld_spinor_raw(typename St<T>::E (&w)[12], const 
     }
 }
 
-template <bool NT, typename T>
+// Row 2 of a special unitary 3x3 matrix from rows 0 and 1 (row-major entries u[0..5]):
+// U[2][j] = conj(eps_jkl U[0][k] U[1][l]), i.e. conj(row 0 x row 1).
+template <typename R>
+__device__ __forceinline__ void third_row(cx<R> (&u)[9]) {
+    const cx<R> a = cmul(u[1], u[5]) - cmul(u[2], u[4]);
+    const cx<R> b = cmul(u[2], u[3]) - cmul(u[0], u[5]);
+    const cx<R> c = cmul(u[0], u[4]) - cmul(u[1], u[3]);
+    u[6] = {a.x, -a.y};
+    u[7] = {b.x, -b.y};
+    u[8] = {c.x, -c.y};
+}
+
+// The 9 entries of the link at site k of a link block.  LF = 9: all entries stored (block of
+// 9S elements).  LF = 6: rows 0 and 1 stored (block of 6S elements, entry e < 6 at the same
+// place as in the full block), row 2 rebuilt by third_row: two-row storage, for links the
+// host has checked to be special unitary to rounding (FP64 and FP32 storage).
+template <bool NT, typename T, int LF = 9>
 __device__ __forceinline__ void ld_link_raw(typename St<T>::E (&u)[9], const typename St<T>::E *base, int S,
                                             int k) {
     constexpr int P = St<T>::pack;
+    static_assert(LF == 9 || (LF == 6 && !St<T>::scaled), "two-row links: FP64 and FP32 storage");
 #pragma unroll
-    for (int q = 0; q < 8 / P; ++q) {
+    for (int q = 0; q < (LF == 9 ? 8 : 6) / P; ++q) {
         const auto v = ld_vec<NT, T>(base + (size_t)q * P * S + (size_t)P * k);
 #pragma unroll
         for (int i = 0; i < P; ++i) u[P * q + i] = v.e[i];
     }
-    u[8] = ldraw<NT>(base + (size_t)8 * S + k);
+    if constexpr (LF == 9)
+        u[8] = ldraw<NT>(base + (size_t)8 * S + k);
+    else
+        third_row(u);
 }
 
 // FP32 scale plane of the rhs block starting at `base` (f16s only).

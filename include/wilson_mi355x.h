@@ -303,7 +303,16 @@ int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, d
  *       kernel by kernel.  Results are identical either way.
  *   WL_OPT_PIPELINE  multi-rhs kernel: 0 (default) = software-pipelined hops where the
  *       registers allow (FP32 and FP16 storage, up to 2 rhs per lane); 1 = never.
- * Results do not depend on these options; only the speed does. */
+ * Results do not depend on these options; only the speed does.
+ *   WL_OPT_LINK_ROWS  3 (default): the FP64 and FP32 hopping kernels read all 9 entries of
+ *       every link.  2: they read rows 0 and 1 and rebuild row 2 as conj(row 0 x row 1), 96
+ *       instead of 144 bytes per FP64 link (a third less link traffic).  Allowed only for
+ *       special unitary links: the uploaded links are checked (when the option is set, and
+ *       after every upload) and every stored third row must agree with the rebuilt one to
+ *       1e-13 in every real part, else WL_E_ARG.  Results then agree with the full-link ones
+ *       to rounding, not bit for bit.  Two-row copies take 2/3 of the gauge field's memory per
+ *       precision used.  FP16 storage always reads full links.  The fermion force reads the
+ *       full links. */
 enum {
     WL_OPT_TRAVERSAL_CHUNK = 1,
     WL_OPT_BLOCK = 2,
@@ -313,7 +322,8 @@ enum {
     WL_OPT_LINK_LOAD = 6,
     WL_OPT_LDS_RESERVE = 7,
     WL_OPT_GRAPHS = 8,
-    WL_OPT_PIPELINE = 9
+    WL_OPT_PIPELINE = 9,
+    WL_OPT_LINK_ROWS = 10
 };
 int wl_set_option(wl_ctx *ctx, int option, int value);
 int wl_get_option(const wl_ctx *ctx, int option, int *value);
