@@ -73,6 +73,43 @@ def solve_series(W, D, fin, fx, n, precision, accuracy=1e-10, delta=1e-3):
     }
 
 
+BYTES_PER_SITE_F64_ROWS = 8 * 96 + 2 * 192   # two-row links: rows 0 and 1 stored (12 reals)
+
+
+def two_row_config(W, lat, D, fin, fodd, x64_full, n_solves):
+    """The same 32^4 field with two-row links (WL_OPT_LINK_ROWS = 2): the hopping term's time
+    and its roofline fraction on the bytes it actually needs (1152 per site), FP64 and
+    FP32-inner CG, and the FP64 multi-rhs hop at 12 rhs, against full links."""
+    vh = lat.Vh
+    out = {"bytes_per_site": BYTES_PER_SITE_F64_ROWS}
+    N = 12
+    f12 = W.Field(lat, W.EVEN, W.F64, N)
+    for j in range(N):
+        f12.copy_rhs_from(j, fin)
+    o12 = W.Field(lat, W.ODD, W.F64, N)
+    ms_full = D.time(fin, fodd, "hop", 1, warmup=5, iters=100)
+    ms12_full = D.time(f12, o12, "hop", 1, warmup=3, iters=20)
+    lat.set_option(W.OPT_LINK_ROWS, 2)
+    try:
+        ms = D.time(fin, fodd, "hop", 1, warmup=5, iters=100)
+        ms12 = D.time(f12, o12, "hop", 1, warmup=3, iters=20)
+        gbs = BYTES_PER_SITE_F64_ROWS * vh / (ms * 1e-3) / 1e9
+        out["dslash_us"] = round(ms * 1e3, 2)
+        out["dslash_us_full_links"] = round(ms_full * 1e3, 2)
+        out["dslash_speedup"] = round(ms_full / ms, 3)
+        out["dslash_gbs"] = round(gbs, 1)
+        out["dslash_hbm_frac"] = round(gbs / HBM_PEAK_GBS, 4)
+        out["multi_rhs_12_us_per_rhs"] = round(ms12 * 1e3 / N, 2)
+        out["multi_rhs_12_us_per_rhs_full_links"] = round(ms12_full * 1e3 / N, 2)
+        x = W.Field(lat, W.EVEN)
+        out["cg"] = solve_series(W, D, fin, x, n_solves, "f64")
+        out["cg"]["solution_rel_diff_vs_full_links"] = rel_diff(x.download(), x64_full.download())
+        out["cg_mixed_f32"] = solve_series(W, D, fin, x, n_solves, "f32")
+    finally:
+        lat.set_option(W.OPT_LINK_ROWS, 3)
+    return out
+
+
 SMALL_DIMS, SMALL_KAPPA = (8, 8, 8, 8), 0.12
 
 
@@ -769,6 +806,10 @@ def main(argv=None):
     if cg_out and cg_mixed:
         cg_mixed["speedup_vs_f64"] = round(cg_mixed["solves_per_s"] / cg_out["solves_per_s"], 3)
         cg_mixed["solution_rel_diff_vs_f64"] = rel_diff(xmx.download(), x64.download())
+    rows = None
+    if a.cg_solves > 0:
+        rows = two_row_config(W, lat, D, fin, fodd, x64, max(2, a.cg_solves // 2))
+        print(f"# two-row links: {rows}", file=sys.stderr, flush=True)
     del x64, xmx
     small = small_cg_config(W, a.cg_solves) if a.cg_solves > 0 else None
     big = multi = serve = None
@@ -819,6 +860,7 @@ def main(argv=None):
         "cg": cg_out,
         "cg_mixed_f32": cg_mixed,
         "cg_8x8x8x8": small,
+        "two_row_links_32x32x32x32": rows,
         "mixed_48x48x48x96": big,
         "multi_rhs_32x32x32x32": multi,
         "serving": serve,
