@@ -285,18 +285,23 @@ int rhs_per_launch(const wl_ctx *c, int nr, int N) {
     return rhs_per_launch_of(c->rhs_launch > 0 ? c->rhs_launch : N, nr, N);
 }
 
-// Geometry of one multi-rhs hopping launch of nb rhs.  FP64 launches of four or more rhs, or
-// of two or more on lattices of 2^17 sites per parity or more, hold every workgroup to 32 KB
-// of LDS per 64 lanes (at most five 64-lane workgroups, 320 sites, per CU; workgroups of 64
-// lanes unless the caller chose a size): at full occupancy the neighbour spinors of the sites
-// in flight overflow the 4 MB L2 of an XCD and are fetched from HBM about three times per
-// hop.  Applies only while the caller has set no LDS reserve and the links are not staged in
-// LDS.
+// Geometry of one multi-rhs hopping launch of nb rhs: an occupancy cap on large launches.
+// FP64 launches of four or more rhs, or of two or more on lattices of 2^17 sites per parity or
+// more, hold every workgroup to 32 KB of LDS per 64 lanes (at most five 64-lane workgroups,
+// 320 sites, per CU); FP32 launches of four or more rhs on lattices of 2^17 sites per parity or
+// more to 24 KB (six).  Workgroups are of 64 lanes unless the caller chose a size.  At full
+// occupancy the neighbour spinors of the sites in flight overflow the 4 MB L2 of an XCD and
+// are fetched from HBM several times per hop.  FP16 (fewer bytes per site) does not gain.
+// Applies only while the caller has set no LDS reserve and the links are not staged in LDS.
 Geom mr_geom(const wl_ctx *c, int prec, int nb, bool lds_links) {
     Geom g = c->g;
-    if (c->lds_auto && !lds_links && prec == WL_F64 && (nb >= 4 || (nb >= 2 && g.vh >= (1 << 17)))) {
+    if (!c->lds_auto || lds_links) return g;
+    int per64 = 0;  // LDS bytes per 64 lanes
+    if (prec == WL_F64 && (nb >= 4 || (nb >= 2 && g.vh >= (1 << 17)))) per64 = 32768;
+    if (prec == WL_F32 && nb >= 4 && g.vh >= (1 << 17)) per64 = 24576;
+    if (per64) {
         if (c->blk_auto) g.blk = 64;
-        g.lds = std::min(65536, 32768 * (g.blk / 64));
+        g.lds = std::min(65536, per64 * (g.blk / 64));
     }
     return g;
 }
@@ -949,9 +954,10 @@ extern "C" int wl_autotune(wl_ctx *c, int prec, int nrhs, wl_tune_result *out) {
         cleanup();
         return rc;
     }
-    // LDS reserve: the caller's if set; else, in FP64, the automatic occupancy cap or none
+    // LDS reserve: the caller's if set; else, in FP64 and FP32, the automatic occupancy cap or
+    // none
     std::vector<int> reserves{saved.lds_reserve};
-    if (saved.lds_reserve == 0 && prec == WL_F64) reserves.push_back(-1);
+    if (saved.lds_reserve == 0 && prec != WL_F16) reserves.push_back(-1);
     const int nr_cap = prec == WL_F64 ? 2 : 4;
     static const int launches[] = {1, 2, 4, 6, 12, 16, 24, 32, 48, 64};
     for (int nr = 1; nr <= std::min(nr_cap, nrhs); nr *= 2) {

@@ -294,8 +294,9 @@ int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, d
  *   WL_OPT_LDS_RESERVE  bytes of LDS reserved by every hopping workgroup (1..65536; -1 =
  *       none; 0 = automatic, the default: none, except multi-rhs FP64 launches of four or
  *       more rhs, or of two or more on lattices of 2^17 sites per parity or more, which reserve
- *       32 KB per 64 lanes and, unless WL_OPT_BLOCK is set, use 64-lane workgroups: at most 5
- *       workgroups per CU).  The
+ *       32 KB per 64 lanes (at most 5 workgroups per CU), and FP32 launches of four or more
+ *       rhs on lattices of 2^17 sites per parity or more, 24 KB (6); both, unless WL_OPT_BLOCK
+ *       is set, in 64-lane workgroups).  The
  *       kernels use no LDS; the reservation caps how many workgroups share a CU, i.e. the
  *       sites in flight, whose neighbour spinors must fit the caches to be reused.
  *   WL_OPT_GRAPHS  1: the batched CG (nrhs > 1, FP16 inner, the server) captures each chunk
@@ -332,8 +333,8 @@ int wl_get_option(const wl_ctx *ctx, int option, int *value);
  * sides at precision prec it times the hopping term on scratch fields of this context for
  * every candidate of rhs per lane (1, 2, 4 in FP32; 1, 2 in FP64), rhs per launch (1, 2, 4, 6,
  * 12, ..., nrhs: the RHS block of one pass over the lattice) and workgroup size (64, 128, 256),
- * and sets WL_OPT_RHS_BLOCK, WL_OPT_RHS_LAUNCH and WL_OPT_BLOCK to the fastest (in FP64, with
- * WL_OPT_LDS_RESERVE automatic, also whether the occupancy cap applies).  The choice is
+ * and sets WL_OPT_RHS_BLOCK, WL_OPT_RHS_LAUNCH and WL_OPT_BLOCK to the fastest (in FP64 and
+ * FP32, with WL_OPT_LDS_RESERVE automatic, also whether the occupancy cap applies).  The choice is
  * remembered per (lattice size, precision, nrhs) for the life of the process, so that later
  * calls, and other contexts of the same size, take the same configuration without timing
  * again (results then do not depend on timing noise).  The gauge field must be uploaded. */

@@ -78,18 +78,19 @@ def test_rhs_block_and_workgroup_options_do_not_change_results():
         lat.set_option(W.OPT_RHS_BLOCK, 5)
 
 
-def test_fp64_occupancy_cap_and_lds_reserve_do_not_change_results():
-    """2^17 sites per parity: FP64 multi-rhs hops take the automatic occupancy cap (64-lane
-    workgroups, 32 KB LDS each); no reserve, an explicit one and a chosen workgroup size give
-    the same bits."""
-    dims, N = (16, 16, 32, 32), 3
+@pytest.mark.parametrize("prec,N", [(W.F64, 3), (W.F32, 5)], ids=["f64", "f32"])
+def test_occupancy_cap_and_lds_reserve_do_not_change_results(prec, N):
+    """2^17 sites per parity: FP64 and FP32 multi-rhs hops take the automatic occupancy cap
+    (64-lane workgroups with 32 / 24 KB of LDS each); no reserve, an explicit one and a chosen
+    workgroup size give the same bits."""
+    dims = (16, 16, 32, 32)
     lat, U = _setup(dims)
     del U
     D = W.Dirac_Wilson_evenodd(0.125, lat)
-    fin = W.Field(lat, EVEN, W.F64, nrhs=N)
+    fin = W.Field(lat, EVEN, prec, nrhs=N)
     for j in range(N):
         fin.upload(W.gen_spinor(dims, 450 + j, EVEN), rhs=j)
-    fe = W.Field(lat, EVEN, W.F64, nrhs=N)
+    fe = W.Field(lat, EVEN, prec, nrhs=N)
     assert lat.get_option(W.OPT_LDS_RESERVE) == 0
     ref = None
     for lds in (0, -1, 32768, 0):
