@@ -713,6 +713,9 @@ def main(argv=None):
     ap.add_argument("--profile-serve", action="store_true",
                     help="short run for rocprofv3: the closed batch of 24 served requests, FP16 "
                          "inner; prints no JSON line")
+    ap.add_argument("--two-rows", action="store_true",
+                    help="with --profile: the hopping launches and the CG read two-row links "
+                         "(WL_OPT_LINK_ROWS = 2)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: --steps hopping launches, then --steps "
                          "plain field copies (12 planes of V/2 complex doubles each way) as "
@@ -750,6 +753,8 @@ def main(argv=None):
     print(f"# setup {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
 
     if a.profile:
+        if a.two_rows:
+            lat.set_option(W.OPT_LINK_ROWS, 2)
         fcopy = W.Field(lat, W.EVEN)
         for _ in range(a.steps):
             D.hop_set(fin, fodd, 1)

@@ -77,7 +77,8 @@ int cg_f64(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int
             L.out = nullptr;
             L.in = t;
             L.xin = Dp;
-            L.gauge = c->gauge64;
+            L.gauge = gauge_for(c, WL_F64);
+            L.lf = link_format(c, WL_F64);
             L.a = kappa;
             L.b = 1.0 + c->op_shift;
             L.out_par = 0;
