@@ -265,18 +265,26 @@ int work_field(wl_ctx *c, int slot, int prec, int parity, int nrhs, wl_field **o
 }
 
 // Right-hand sides per lane of the multi-rhs kernel (register blocking): at most 2 in FP64
-// (3-4 spill) and 4 with FP32 arithmetic.  Defaults from the served request mix on MI355X:
-// 4 for FP32 storage, 2 for FP64 and FP16 (FP16 at 4 per lane spills in the pipelined form).
+// (3-4 spill) and 4 with FP32 arithmetic.  Defaults, from wl_autotune over {16^4, 24^3x48,
+// 32^4} x 1-12 rhs on MI355X: FP16 one (the packed arithmetic leaves registers for occupancy,
+// which the gathers need more than link reuse) up to 3 x 2^20 site-rhs per launch, two beyond;
+// FP32 two on lattices of 2^17 sites per parity or more from 4 rhs, else one; FP64 two.
 int rhs_per_lane(const wl_ctx *c, int prec, int N) {
     const int cap = prec == WL_F64 ? 2 : 4;
-    const int nr = c->rhs_block > 0 ? c->rhs_block : (prec == WL_F32 ? 4 : 2);
+    int nr = c->rhs_block;
+    if (nr <= 0) {
+        if (prec == WL_F16) nr = (int64_t)N * c->g.vh >= 3 * (int64_t(1) << 20) ? 2 : 1;
+        else if (prec == WL_F32) nr = N >= 4 && c->g.vh >= (1 << 17) ? 2 : 1;
+        else nr = 2;
+    }
     return std::max(1, std::min(nr, std::min(N, cap)));
 }
 
-// Right-hand sides per launch (a multiple of the per-lane count): each launch is one pass over
-// the lattice, so a smaller group keeps the neighbour-spinor footprint of a segment small
-// enough for the XCD L2, at one extra link read per pass.
+// Right-hand sides per launch (a multiple of the per-lane count, or all of them): each launch
+// is one pass over the lattice, so a smaller group keeps the neighbour-spinor footprint of a
+// segment small enough for the XCD L2, at one extra link read per pass.
 int rhs_per_launch_of(int want, int nr, int N) {
+    if (want >= N) return N;  // one pass; its last chunk may be partly filled
     const int nl = std::max(nr, want / nr * nr);
     return std::min(nl, N);
 }
@@ -285,23 +293,28 @@ int rhs_per_launch(const wl_ctx *c, int nr, int N) {
     return rhs_per_launch_of(c->rhs_launch > 0 ? c->rhs_launch : N, nr, N);
 }
 
-// Geometry of one multi-rhs hopping launch of nb rhs: an occupancy cap on large launches.
-// FP64 launches of four or more rhs, or of two or more on lattices of 2^17 sites per parity or
-// more, hold every workgroup to 32 KB of LDS per 64 lanes (at most five 64-lane workgroups,
-// 320 sites, per CU); FP32 launches of four or more rhs on lattices of 2^17 sites per parity or
-// more to 24 KB (six).  Workgroups are of 64 lanes unless the caller chose a size.  At full
-// occupancy the neighbour spinors of the sites in flight overflow the 4 MB L2 of an XCD and
-// are fetched from HBM several times per hop.  FP16 (fewer bytes per site) does not gain.
+// Geometry of one multi-rhs hopping launch of nb rhs, nr per lane.
+// Occupancy cap on large launches: FP64 launches of four or more rhs, or of two or more on
+// lattices of 2^17 sites per parity or more, hold every workgroup to 32 KB of LDS per 64
+// lanes (at most five 64-lane workgroups, 320 sites, per CU); FP32 launches of four or more
+// rhs at three or four per lane, on lattices of 2^17 sites per parity or more, to 24 KB (six).
+// Capped launches use 64-lane workgroups unless the caller chose a size.  At full occupancy
+// the neighbour spinors of the sites in flight overflow the 4 MB L2 of an XCD and are fetched
+// from HBM several times per hop.  FP16, and FP32 at one or two per lane, do not gain.
 // Applies only while the caller has set no LDS reserve and the links are not staged in LDS.
-Geom mr_geom(const wl_ctx *c, int prec, int nb, bool lds_links) {
+// Uncapped FP32 and FP16 launches of two or more rhs default to 256-lane workgroups.
+Geom mr_geom(const wl_ctx *c, int prec, int nb, int nr, bool lds_links) {
     Geom g = c->g;
-    if (!c->lds_auto || lds_links) return g;
     int per64 = 0;  // LDS bytes per 64 lanes
-    if (prec == WL_F64 && (nb >= 4 || (nb >= 2 && g.vh >= (1 << 17)))) per64 = 32768;
-    if (prec == WL_F32 && nb >= 4 && g.vh >= (1 << 17)) per64 = 24576;
+    if (c->lds_auto && !lds_links) {
+        if (prec == WL_F64 && (nb >= 4 || (nb >= 2 && g.vh >= (1 << 17)))) per64 = 32768;
+        if (prec == WL_F32 && nr >= 3 && nb >= 4 && g.vh >= (1 << 17)) per64 = 24576;
+    }
     if (per64) {
         if (c->blk_auto) g.blk = 64;
         g.lds = std::min(65536, per64 * (g.blk / 64));
+    } else if (c->blk_auto && prec != WL_F64 && nb >= 2) {
+        g.blk = 256;
     }
     return g;
 }
@@ -342,7 +355,7 @@ int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, c
         L.p = p;
         L.r = r;
         L.partials = c->partials;
-        HIPCHK(launch_dslash_mr(prec, mr_geom(c, prec, L.NB, L.lds_links), L, c->stream));
+        HIPCHK(launch_dslash_mr(prec, mr_geom(c, prec, L.NB, L.NR, L.lds_links), L, c->stream));
     }
     if (pstride_out) *pstride_out = pstride;
     return WL_OK;
@@ -884,8 +897,8 @@ int lds_reserve_of(const wl_ctx *c) { return c->lds_auto ? 0 : c->g.lds > 0 ? c-
 void apply_hop_config(wl_ctx *c, const wl_tune_result &r) {
     c->rhs_block = r.rhs_block;
     c->rhs_launch = r.rhs_launch;
-    c->g.blk = r.block;
-    c->blk_auto = false;
+    c->blk_auto = r.block == 0;
+    c->g.blk = r.block == 0 ? dslash_block_auto(c->g.vh) : r.block;
     set_lds_reserve(c, r.lds_reserve);
 }
 }  // namespace wl
@@ -954,10 +967,11 @@ extern "C" int wl_autotune(wl_ctx *c, int prec, int nrhs, wl_tune_result *out) {
         cleanup();
         return rc;
     }
-    // LDS reserve: the caller's if set; else, in FP64 and FP32, the automatic occupancy cap or
-    // none
-    std::vector<int> reserves{saved.lds_reserve};
-    if (saved.lds_reserve == 0 && prec != WL_F16) reserves.push_back(-1);
+    // LDS reserve: automatic (the occupancy cap where it applies), in FP64 and FP32 also none,
+    // and a reserve the caller set
+    std::vector<int> reserves{0};
+    if (prec != WL_F16) reserves.push_back(-1);
+    if (saved.lds_reserve > 0) reserves.push_back(saved.lds_reserve);
     const int nr_cap = prec == WL_F64 ? 2 : 4;
     static const int launches[] = {1, 2, 4, 6, 12, 16, 24, 32, 48, 64};
     for (int nr = 1; nr <= std::min(nr_cap, nrhs); nr *= 2) {
@@ -990,9 +1004,33 @@ extern "C" int wl_autotune(wl_ctx *c, int prec, int nrhs, wl_tune_result *out) {
                     }
                 }
     }
+    // The search times each candidate once, briefly: time the choice and the default
+    // configuration again, alternately, and keep the default unless the choice beats it.
+    wl_tune_result def{};  // rhs per lane, per launch and workgroup automatic, LDS automatic
+    std::vector<double> t_best, t_def;
+    for (int rep = 0; rep < 3; ++rep)
+        for (const wl_tune_result *cfg : {&best, &def}) {
+            apply_hop_config(c, *cfg);
+            double us = 0;
+            if ((rc = time_hop_mr(c, prec, fout->data, fin->data, nrhs, 2 * iters, e0, e1, &us))) {
+                restore();
+                cleanup();
+                return rc;
+            }
+            (cfg == &best ? t_best : t_def).push_back(us / nrhs);
+        }
     cleanup();
+    std::sort(t_best.begin(), t_best.end());
+    std::sort(t_def.begin(), t_def.end());
+    if (t_def[1] <= t_best[1]) {
+        def.us_per_rhs = t_def[1];
+        best = def;
+    } else {
+        best.us_per_rhs = t_best[1];
+    }
     best.candidates = count;
-    best.us_per_rhs_default = us_def / nrhs;
+    best.us_per_rhs_default = t_def[1];
+    (void)us_def;
     apply_hop_config(c, best);
     {
         std::lock_guard<std::mutex> g(g_tune_mu);

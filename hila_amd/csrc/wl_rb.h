@@ -137,7 +137,7 @@ struct RbBuf {
 
 template <typename T, int NR>
 constexpr bool rb_pipe_ok() {
-    return sizeof(typename St<T>::R) == 4 && NR <= 2;  // FP16 with 4 per lane spills
+    return sizeof(typename St<T>::R) == 4 && (NR <= 2 || St<T>::scaled);
 }
 
 template <int NR, bool NTL, typename T, int LF>

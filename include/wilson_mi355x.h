@@ -276,10 +276,12 @@ int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, d
  *   WL_OPT_TRAVERSAL_CHUNK  order in which the 64-site segments of the checkerboard are
  *       visited: 0 = memory order; n > 0 = chunks of n segments of one time slice, visited for
  *       every t before the next chunk (n must divide the segments per slice); -1 = default.
- *   WL_OPT_BLOCK  workgroup size 64, 128 or 256; 0 = default for the volume.
+ *   WL_OPT_BLOCK  workgroup size 64, 128 or 256; 0 = default for the volume (multi-rhs FP32
+ *       and FP16 launches of two or more rhs: 256).
  *   WL_OPT_RHS_BLOCK  right-hand sides per lane of the multi-rhs kernel (register blocking,
- *       1..4, at most 2 in FP64; 0 = default: 4 for FP32 storage, 2 for FP64 and FP16).  A
- *       lane loads each link once for them.
+ *       1..4, at most 2 in FP64; 0 = default: for FP16 storage 1, or 2 when rhs x sites per
+ *       parity reaches 3 x 2^20; for FP32 2 from 4 rhs on lattices of 2^17 sites per parity or
+ *       more, else 1; 2 for FP64).  A lane loads each link once for them.
  *   WL_OPT_RHS_LAUNCH  right-hand sides per launch, i.e. per pass over the lattice (rounded
  *       to a multiple of the per-lane count; 0 = default).  Fewer per pass keep the neighbour
  *       spinors of a pass in cache; each pass reads the links again.
@@ -295,8 +297,8 @@ int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, d
  *       none; 0 = automatic, the default: none, except multi-rhs FP64 launches of four or
  *       more rhs, or of two or more on lattices of 2^17 sites per parity or more, which reserve
  *       32 KB per 64 lanes (at most 5 workgroups per CU), and FP32 launches of four or more
- *       rhs on lattices of 2^17 sites per parity or more, 24 KB (6); both, unless WL_OPT_BLOCK
- *       is set, in 64-lane workgroups).  The
+ *       rhs at 3-4 per lane on lattices of 2^17 sites per parity or more, 24 KB (6); both,
+ *       unless WL_OPT_BLOCK is set, in 64-lane workgroups).  The
  *       kernels use no LDS; the reservation caps how many workgroups share a CU, i.e. the
  *       sites in flight, whose neighbour spinors must fit the caches to be reused.
  *   WL_OPT_GRAPHS  1: the batched CG (nrhs > 1, FP16 inner, the server) captures each chunk
@@ -334,12 +336,14 @@ int wl_get_option(const wl_ctx *ctx, int option, int *value);
  * every candidate of rhs per lane (1, 2, 4 in FP32; 1, 2 in FP64), rhs per launch (1, 2, 4, 6,
  * 12, ..., nrhs: the RHS block of one pass over the lattice) and workgroup size (64, 128, 256),
  * and sets WL_OPT_RHS_BLOCK, WL_OPT_RHS_LAUNCH and WL_OPT_BLOCK to the fastest (in FP64 and
- * FP32, with WL_OPT_LDS_RESERVE automatic, also whether the occupancy cap applies).  The choice is
+ * FP32 also WL_OPT_LDS_RESERVE: automatic, i.e. the occupancy cap where it applies, or none).
+ * The choice is timed again against the default configuration, which is kept unless the choice
+ * is faster.  The choice is
  * remembered per (lattice size, precision, nrhs) for the life of the process, so that later
  * calls, and other contexts of the same size, take the same configuration without timing
  * again (results then do not depend on timing noise).  The gauge field must be uploaded. */
 typedef struct wl_tune_result {
-    int rhs_block;
+    int rhs_block;            /* 0 in all three: the default configuration was fastest      */
     int rhs_launch;
     int block;
     int candidates;           /* configurations timed (0 when the choice was remembered)     */
