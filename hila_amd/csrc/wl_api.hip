@@ -268,14 +268,19 @@ int work_field(wl_ctx *c, int slot, int prec, int parity, int nrhs, wl_field **o
 // (3-4 spill) and 4 with FP32 arithmetic.  Defaults, from wl_autotune over {16^4, 24^3x48,
 // 32^4} x 1-12 rhs on MI355X: FP16 one (the packed arithmetic leaves registers for occupancy,
 // which the gathers need more than link reuse) up to 3 x 2^20 site-rhs per launch, two beyond;
-// FP32 two on lattices of 2^17 sites per parity or more from 4 rhs, else one; FP64 two.
+// FP32 two on lattices of 2^17 sites per parity or more from 4 rhs (from 2 at 2^19 or more),
+// else one; FP64 two, except one for 2-3 rhs below 2^17 sites per parity.
 int rhs_per_lane(const wl_ctx *c, int prec, int N) {
     const int cap = prec == WL_F64 ? 2 : 4;
     int nr = c->rhs_block;
     if (nr <= 0) {
-        if (prec == WL_F16) nr = (int64_t)N * c->g.vh >= 3 * (int64_t(1) << 20) ? 2 : 1;
-        else if (prec == WL_F32) nr = N >= 4 && c->g.vh >= (1 << 17) ? 2 : 1;
-        else nr = 2;
+        const int vh = c->g.vh;
+        if (prec == WL_F16)
+            nr = (int64_t)N * vh >= 3 * (int64_t(1) << 20) ? 2 : 1;
+        else if (prec == WL_F32)
+            nr = (N >= 4 && vh >= (1 << 17)) || (N == 2 && vh >= (1 << 19)) ? 2 : 1;
+        else
+            nr = N <= 3 && vh < (1 << 17) ? 1 : 2;
     }
     return std::max(1, std::min(nr, std::min(N, cap)));
 }

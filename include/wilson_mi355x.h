@@ -281,7 +281,8 @@ int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, d
  *   WL_OPT_RHS_BLOCK  right-hand sides per lane of the multi-rhs kernel (register blocking,
  *       1..4, at most 2 in FP64; 0 = default: for FP16 storage 1, or 2 when rhs x sites per
  *       parity reaches 3 x 2^20; for FP32 2 from 4 rhs on lattices of 2^17 sites per parity or
- *       more, else 1; 2 for FP64).  A lane loads each link once for them.
+ *       more (from 2 rhs at 2^19 or more), else 1; for FP64 2, except 1 for up to 3 rhs below
+ *       2^17 sites per parity).  A lane loads each link once for them.
  *   WL_OPT_RHS_LAUNCH  right-hand sides per launch, i.e. per pass over the lattice (rounded
  *       to a multiple of the per-lane count; 0 = default).  Fewer per pass keep the neighbour
  *       spinors of a pass in cache; each pass reads the links again.
