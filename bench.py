@@ -397,6 +397,22 @@ def serving_config(W, n_poisson=60, n_closed=24, rate=20.0, per_volume=3, seed=1
             raise RuntimeError(f"served {prec}: requests {bad} missed their tolerance")
         print(f"# served closed batch ({prec}): {n_closed} requests, {rhs_c} rhs in "
               f"{closed_wall[prec]:.2f} s", file=sys.stderr, flush=True)
+    # the same closed batch on a server whose slots read two-row links (WL_OPT_LINK_ROWS = 2;
+    # the request links are special unitary): the FP64 reliable updates and the FP32 inner
+    # operator read a third less link data
+    S2 = W.Server(max_active=16)
+    S2.set_option(W.OPT_LINK_ROWS, 2)
+    for prec in SERVE_PRECISIONS:
+        run_served(W, S2, warm, gauges, sources, pool, precision=prec)
+    closed_wall_rows = {}
+    for prec in SERVE_PRECISIONS:
+        res_r, closed_wall_rows[prec] = run_served(W, S2, closed, gauges, sources, pool, precision=prec)
+        bad = [i for i, (r, q) in enumerate(zip(res_r, closed)) if r["true_resid"] > q["tol"]]
+        if bad:
+            raise RuntimeError(f"served {prec} with two-row links: requests {bad} missed their tolerance")
+        print(f"# served closed batch ({prec}, two-row links): {closed_wall_rows[prec]:.2f} s", file=sys.stderr,
+              flush=True)
+    S2.close()
     best = min(closed_wall, key=closed_wall.get)
     wall_c = closed_wall[best]
     res_p, wall_p = run_served(W, S, poisson, gauges, sources, pool, arrivals=arrivals,
@@ -441,6 +457,9 @@ def serving_config(W, n_poisson=60, n_closed=24, rate=20.0, per_volume=3, seed=1
             "plain_f64_single_rhs_s": round(wall_plain, 3),
             "plain_f64_rhs_solves_per_s": round(rhs_c / wall_plain, 2),
             "speedup_vs_plain_f64": {p: round(wall_plain / w, 3) for p, w in closed_wall.items()},
+            "two_row_links_served_s": {p: round(w, 3) for p, w in closed_wall_rows.items()},
+            "two_row_links_speedup_vs_plain_f64": {p: round(wall_plain / w, 3)
+                                                   for p, w in closed_wall_rows.items()},
         },
         "gauge_pool": f"{per_volume} gauge fields per lattice size (each with its own eps and "
                       f"seed), 12 Gaussian sources per size; a request draws one gauge field "

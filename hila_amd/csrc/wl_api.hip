@@ -195,6 +195,7 @@ int dslash_one(wl_ctx *c, int prec, void *out, int out_par, const void *in, cons
     L.xin = xin;
     L.gauge = gauge_for(c, prec);
     L.lf = link_format(c, prec);
+    if (!L.gauge) return fail(WL_E_STATE, "links of precision %d not prepared (ensure_gauge)", prec);
     L.a = a;
     L.b = 1.0 + c->op_shift;
     L.out_par = out_par;
@@ -332,6 +333,7 @@ int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, c
         int rc = ensure_partials(c, (size_t)N * pstride);
         if (rc) return rc;
     }
+    if (!gauge_for(c, prec)) return fail(WL_E_STATE, "links of precision %d not prepared (ensure_gauge)", prec);
     const int nr = rhs_per_lane(c, prec, N);
     const int nl = rhs_per_launch(c, nr, N);
     for (int j0 = 0; j0 < N; j0 += nl) {

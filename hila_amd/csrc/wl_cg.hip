@@ -79,6 +79,7 @@ int cg_f64(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int
             L.xin = Dp;
             L.gauge = gauge_for(c, WL_F64);
             L.lf = link_format(c, WL_F64);
+            if (!L.gauge) return fail(WL_E_STATE, "FP64 links not prepared");
             L.a = kappa;
             L.b = 1.0 + c->op_shift;
             L.out_par = 0;
@@ -203,6 +204,7 @@ int cg_mixed(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, i
             L.xin = Dp;
             L.gauge = gauge_for(c, prec);
             L.lf = link_format(c, prec);
+            if (!L.gauge) return fail(WL_E_STATE, "links of precision %d not prepared", prec);
             L.a = kappa;
             L.b = 1.0 + c->op_shift;
             L.out_par = 0;
@@ -265,7 +267,10 @@ void BatchCG::drop_graphs() {
 
 int BatchCG::prepare(wl_ctx *c, int N, int prec, bool mixed) {
     int rc;
+    // the inner operator's links, and the FP64 ones of the reliable updates and true residuals
+    // (a separate copy when two-row links are on)
     if ((rc = ensure_gauge(c, prec))) return rc;
+    if (prec != WL_F64 && (rc = ensure_gauge(c, WL_F64))) return rc;
     if ((rc = ensure_st_mr(c, N))) return rc;
     wl_field *f;
     if ((rc = work_field(c, W_R, prec, WL_EVEN, N, &f)) || (rc = work_field(c, W_P, prec, WL_EVEN, N, &f)) ||

@@ -544,7 +544,7 @@ extern "C" int wl_server_info_get(wl_server *S, wl_server_info *info) {
 
 extern "C" int wl_server_set_option(wl_server *S, int option, int value) {
     if (!S) return fail(WL_E_ARG, "null server");
-    if (option < WL_OPT_TRAVERSAL_CHUNK || option > WL_OPT_PIPELINE) return fail(WL_E_ARG, "unknown option %d", option);
+    if (option < WL_OPT_TRAVERSAL_CHUNK || option > WL_OPT_LINK_ROWS) return fail(WL_E_ARG, "unknown option %d", option);
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->submitted) return fail(WL_E_STATE, "server options must be set before the first submission");
     S->slot_opts.emplace_back(option, value);

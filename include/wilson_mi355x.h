@@ -440,7 +440,8 @@ int wl_server_info_get(wl_server *srv, wl_server_info *info);
 /* A kernel option (WL_OPT_*, as wl_set_option) for every slot of the server; only before the
  * first submission (else WL_E_STATE).  The multi-rhs kernel shape (WL_OPT_RHS_BLOCK,
  * WL_OPT_RHS_LAUNCH, WL_OPT_BLOCK) is taken per request from wl_autotune's remembered choice
- * for its lattice size, precision and rhs count, or the defaults. */
+ * for its lattice size, precision and rhs count, or the defaults.  With WL_OPT_LINK_ROWS = 2 a
+ * request whose links are not special unitary to 1e-13 completes with status WL_E_ARG. */
 int wl_server_set_option(wl_server *srv, int option, int value);
 /* Finish every submitted request, then release all device memory and threads. */
 int wl_server_destroy(wl_server *srv);

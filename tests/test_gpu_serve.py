@@ -196,3 +196,35 @@ def test_server_with_graph_replay_matches():
     for ((res, st), sol), (bres, bst, bsol) in zip(outs, base):
         assert res["status"] == W.WL_OK
         assert np.array_equal(sol, bsol)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_two_row_links_server_matches_blocking_solver(prec):
+    """WL_OPT_LINK_ROWS = 2 on every slot: the same bits as the blocking solver with two-row
+    links, within the tolerance on the FP64 true residual; links that are not special unitary
+    are refused with WL_E_ARG and do not disturb the next request."""
+    r = Req((8, 8, 8, 8), 21, 3, 0.124, 1e-10, prec, 0.3)
+    bad = Req((8, 8, 8, 8), 22, 1, 0.124, 1e-10, prec, 0.3)
+    bad.U[1, 5, :] *= 1.001
+    with W.Server(max_active=2) as S:
+        S.set_option(W.OPT_LINK_ROWS, 2)
+        tb, _ = bad.submit(S)
+        resb, _ = S.wait(tb)
+        t, sol = r.submit(S)
+        res, st = S.wait(t)
+    assert resb["status"] == W.WL_E_ARG
+    assert res["status"] == W.WL_OK
+    lat = W.Lattice(r.dims)
+    lat.upload_gauge(r.U)
+    lat.set_option(W.OPT_LINK_ROWS, 2)
+    D = W.Dirac_Wilson_evenodd(r.kappa, lat)
+    b = W.Field(lat, EVEN, W.F64, nrhs=r.nrhs)
+    x = W.Field(lat, EVEN, W.F64, nrhs=r.nrhs)
+    for j in range(r.nrhs):
+        b.upload(r.src[j], rhs=j)
+    bst = W.CG(D, accuracy=r.tol, maxiters=10000, precision=prec, delta=0.1, update_every=100).apply(b, x)
+    for j in range(r.nrhs):
+        assert np.array_equal(x.download(j), sol[j]), j
+        assert bst[j]["iterations"] == st[j]["iterations"]
+        assert true_resid(r, sol, j) <= 1.5 * r.tol   # FP64 CG stops on its iterated residual
+    lat.close()
