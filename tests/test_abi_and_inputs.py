@@ -35,6 +35,24 @@ def test_library_exports_every_declared_symbol():
     assert set(W.EXPORTED_SYMBOLS) <= set(names), "binding lists a symbol the header lacks"
 
 
+def test_python_constants_match_the_header():
+    """Every WL_* enumerator of include/wilson_mi355x.h has the same value in the Python
+    mirror (status codes, options, precisions, parities)."""
+    import os
+    import re
+    hdr = open(os.path.join(os.path.dirname(W.__file__), "..", "include", "wilson_mi355x.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)            # comments mention values too
+    hdr = re.sub(r"//[^\n]*", "", hdr)
+    enums = {m.group(1): int(m.group(2)) for m in re.finditer(r"\b(WL_[A-Z0-9_]+)\s*=\s*(-?\d+)", hdr)}
+    assert len(enums) >= 20
+    for name, value in enums.items():
+        py = name[3:] if name.startswith(("WL_OPT_", "WL_F", "WL_EVEN", "WL_ODD")) else name
+        if hasattr(W, py):
+            assert getattr(W, py) == value, name
+        else:
+            assert hasattr(W, name) and getattr(W, name) == value, name
+
+
 def test_version_and_error_reporting():
     assert b"gfx950" in W.lib().wl_version()
     with pytest.raises(W.WilsonError) as e:
