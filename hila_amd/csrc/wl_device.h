@@ -299,10 +299,11 @@ __device__ __forceinline__ SiteNb neighbours(const Geom &g, int k, int p) {
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b -> XCD b % 8).  Map them so
 // that each XCD walks one contiguous eighth of the checkerboard: neighbouring sites (and the
-// spinors they share) then stay in one XCD's L2.
+// spinors they share) then stay in one XCD's L2.  XCD x takes nb / 8 tiles, one more when
+// x < nb % 8; a bijection of [0, nb) for every nb.
 __device__ __forceinline__ int xcd_tile(int b, int nb) {
-    if (nb & 7) return b;
-    return (b & 7) * (nb >> 3) + (b >> 3);
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
 }
 
 // Checkerboard site index handled by this lane of a hopping-kernel launch (>= vh: idle lane).
