@@ -258,3 +258,24 @@ def test_field_outliving_its_context_is_orphaned():
         f.download()
     assert e.value.code == W.WL_E_STATE
     f.close()
+
+
+@pytest.mark.parametrize("dims", [(6, 6, 6, 10), (10, 10, 10, 12), (20, 20, 20, 20)],
+                         ids=lambda d: "x".join(map(str, d)))
+def test_workgroup_counts_that_are_not_multiples_of_eight(dims):
+    """Lattices whose hopping launches have a workgroup count that is not a multiple of 8
+    (the XCD-aware workgroup map then gives some XCDs one tile more): single- and multi-rhs
+    operator against the oracle, for each storage format."""
+    print("odd workgroup counts", dims, flush=True)
+    lat, U, orc = _setup(dims, "smooth", 0.3, 61, True)
+    D = W.Dirac_Wilson_evenodd(0.12, lat)
+    srcs = [W.gen_spinor(dims, 620 + j, EVEN) for j in range(3)]
+    for prec, tol in ((W.F64, 1e-13), (W.F32, 2e-6), (W.F16, 2e-3)):
+        for N in (1, 3):
+            fin, fe = W.Field(lat, EVEN, prec, N), W.Field(lat, EVEN, prec, N)
+            for j in range(N):
+                fin.upload(srcs[j], rhs=j)
+            D.apply(fin, fe)
+            for j in range(N):
+                assert rel(fe.to_full(j), orc.meo(U, 0.12, srcs[j], 1)) <= tol, (prec, N, j)
+    lat.close()
