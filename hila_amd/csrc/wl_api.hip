@@ -962,14 +962,14 @@ extern "C" int wl_autotune(wl_ctx *c, int prec, int nrhs, wl_tune_result *out) {
         cleanup();
         return fail(WL_E_DEVICE, "event creation failed");
     }
-    // default configuration first
+    // warm-up: the default configuration (timed again against the choice at the end)
     c->rhs_block = 0;
     c->rhs_launch = 0;
     c->g.blk = dslash_block_auto(c->g.vh);
     c->blk_auto = true;
     set_lds_reserve(c, 0);
-    double us_def = 0;
-    if ((rc = time_hop_mr(c, prec, fout->data, fin->data, nrhs, iters, e0, e1, &us_def))) {
+    double us_warm = 0;
+    if ((rc = time_hop_mr(c, prec, fout->data, fin->data, nrhs, iters, e0, e1, &us_warm))) {
         restore();
         cleanup();
         return rc;
@@ -1037,7 +1037,6 @@ extern "C" int wl_autotune(wl_ctx *c, int prec, int nrhs, wl_tune_result *out) {
     }
     best.candidates = count;
     best.us_per_rhs_default = t_def[1];
-    (void)us_def;
     apply_hop_config(c, best);
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
