@@ -54,6 +54,14 @@ class Lattice {
     int64_t volume() const { return volume_; }
     const std::array<int, 4> &size() const { return size_; }
     void synchronize() { check(wl_sync(ctx_), "wl_sync"); }
+    // kernel options (WL_OPT_*, wl_set_option): launch shapes, or WL_OPT_LINK_ROWS = 2 for
+    // two-row links on special unitary gauge fields
+    void set_option(int option, int value) { check(wl_set_option(ctx_, option, value), "wl_set_option"); }
+    int option(int option) const {
+        int v = 0;
+        check(wl_get_option(ctx_, option, &v), "wl_get_option");
+        return v;
+    }
 
   private:
     std::array<int, 4> size_;

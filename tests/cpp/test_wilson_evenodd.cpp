@@ -106,6 +106,23 @@ static void run(Lattice &lattice, int link_kind, const char *name) {
         CHECK(diffre < 1e-14 * b.squarenorm(EVEN), "mixed (DdgD)^-1 DdgD");
     }
 
+    // two-row links (special unitary links: unit and Haar): the same round trip, the operator
+    // within 1e-13 of the full-link one; then back to full links
+    {
+        lattice.set_option(WL_OPT_LINK_ROWS, 2);
+        CHECK(lattice.option(WL_OPT_LINK_ROWS) == 2, "two-row links on");
+        WilsonField Db2(lattice);
+        D.apply(b, Db2);
+        CHECK(diff2_even(Db2, Db) < 1e-26 * Db.squarenorm(EVEN), "two-row operator = full-link operator");
+        CG<dirac> rows(D, 1e-10, 10000);
+        a.zero(ALL);
+        rows.apply(DdaggerDb, a);
+        diffre = diff2_even(a, b);
+        CHECK(rows.stats.converged == 1 && rows.stats.true_resid <= 1e-10 && diffre < 1e-16 * b.squarenorm(EVEN),
+              "CG with two-row links");
+        lattice.set_option(WL_OPT_LINK_ROWS, 3);
+    }
+
     // GaugeField::config_write then config_read (gaugefield.h:194-272, as checkpoint.h:30,89)
     // gives back the same links, and the operator on them the same output, to the last bit
     char path[] = "/tmp/wl_config_XXXXXX";
