@@ -687,11 +687,19 @@ def cpu_baseline(W, U, psi, seconds=8.0):
 
     small = cg(SMALL_DIMS, W.gen_gauge(SMALL_DIMS, "haar", 0, 1), SMALL_KAPPA,
                W.gen_spinor(SMALL_DIMS, 2, W.EVEN))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
     big = cg(DIMS, U, KAPPA, psi)
     return {
         "value": round(BYTES_PER_SITE_F64 * vh * n / el / 1e9, 3),
         "unit": "GB/s",
         "cores": cores,
+        "cpu_model": model,
+        "host_cpus": os.cpu_count(),
         "kind": "port",
         "sample": f"{n} half-lattice hopping applications (Dirac_Wilson_hop_set restated, "
                   f"HILA pass structure) at 32^4 FP64, {el:.1f} s, OpenMP {cores} threads",
