@@ -646,13 +646,16 @@ extern "C" int wl_field_copy(wl_field *dst, const wl_field *src) {
     if (!dst || !src) return fail(WL_E_ARG, "null argument");
     if (!dst->ctx || !src->ctx) return fail(WL_E_STATE, "the field's context was destroyed");
     if (dst->ctx != src->ctx) return fail(WL_E_ARG, "fields belong to different contexts");
-    if (dst->parity != src->parity || dst->prec != src->prec || dst->nrhs != src->nrhs)
-        return fail(WL_E_ARG, "fields differ in parity, precision or nrhs");
+    if (dst->parity != src->parity || dst->nrhs != src->nrhs)
+        return fail(WL_E_ARG, "fields differ in parity or nrhs");
     if (dst == src) return WL_OK;
     wl_ctx *c = dst->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
     DeviceGuard dg(c->device);
-    HIPCHK(launch_copy(src->prec, c->g, src->data, dst->data, src->nrhs, c->stream));
+    if (dst->prec == src->prec)
+        HIPCHK(launch_copy(src->prec, c->g, src->data, dst->data, src->nrhs, c->stream));
+    else
+        HIPCHK(launch_convert(src->prec, dst->prec, c->g, src->data, dst->data, src->nrhs, c->stream));
     return WL_OK;
 }
 

@@ -116,9 +116,15 @@ __global__ __launch_bounds__(kBlock) void k_two_rows(const cx<double> *__restric
 #pragma unroll
             for (int e = 0; e < 9; ++e) w[e] = u[e];
             third_row(w);
+            // fmax would drop a NaN operand: every difference is tested on its own, and a NaN
+            // (from a NaN or infinite entry in any row) makes the deviation infinite
 #pragma unroll
-            for (int e = 6; e < 9; ++e) d = fmax(d, fmax(fabs(w[e].x - u[e].x), fabs(w[e].y - u[e].y)));
-            if (d != d) d = INFINITY;  // NaN entries fail the check
+            for (int e = 6; e < 9; ++e)
+#pragma unroll
+                for (int part = 0; part < 2; ++part) {
+                    const double x = part ? fabs(w[e].y - u[e].y) : fabs(w[e].x - u[e].x);
+                    if (!(x <= d)) d = (x != x) ? INFINITY : x;
+                }
         }
     }
     if (dev) {  // grid-uniform: every thread reaches the barriers
@@ -238,6 +244,27 @@ __global__ __launch_bounds__(kBlock) void k_copy_words(const unsigned *__restric
                                                        unsigned *__restrict__ dst, size_t n) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) dst[i] = src[i];
+}
+
+// Right-hand side blockIdx.y of dst = the same of src in another storage format: HILA's
+// assignment between fields of different element precision (c[ALL] = chi[X] with c a float
+// field, libraries/hmc/fermion_field.h:77-78).  Each component is rounded once to the
+// destination's arithmetic type; FP16 storage then takes its site scale as st_site does.
+template <typename Ti, typename To>
+__global__ __launch_bounds__(kBlock) void k_convert(const typename St<Ti>::E *__restrict__ in,
+                                                    typename St<To>::E *__restrict__ out, int vh, int S) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= vh) return;
+    const size_t j = blockIdx.y;
+    cx<typename St<Ti>::R> v[4][3];
+    ld_site<Ti>(in + j * St<Ti>::planes * S, S, k, v);
+    using Ro = typename St<To>::R;
+    cx<Ro> w[4][3];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) w[s][c] = {(Ro)v[s][c].x, (Ro)v[s][c].y};
+    st_site<To>(out + j * St<To>::planes * S, S, k, w);
 }
 
 // Sum the n partials in workgroup order and apply `fin` (skipped once the CG has finished).
@@ -406,6 +433,29 @@ hipError_t launch_copy(int prec, const Geom &g, const void *src, void *dst, int 
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+template <typename Ti>
+static hipError_t convert_from(int pout, const Geom &g, const void *src, void *dst, int nrhs, hipStream_t s) {
+    const dim3 grid((unsigned)nblocks(g), (unsigned)nrhs);
+    const auto *in = (const typename St<Ti>::E *)src;
+    if (pout == WL_F64)
+        hipLaunchKernelGGL((k_convert<Ti, double>), grid, dim3(kBlock), 0, s, in, (cx<double> *)dst, g.vh, g.stride);
+    else if (pout == WL_F32)
+        hipLaunchKernelGGL((k_convert<Ti, float>), grid, dim3(kBlock), 0, s, in, (cx<float> *)dst, g.vh, g.stride);
+    else if (pout == WL_F16)
+        hipLaunchKernelGGL((k_convert<Ti, f16s>), grid, dim3(kBlock), 0, s, in, (h2 *)dst, g.vh, g.stride);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(int pin, int pout, const Geom &g, const void *src, void *dst, int nrhs, hipStream_t s) {
+    if (nrhs < 1 || nrhs > 65535) return hipErrorInvalidValue;
+    if (pin == WL_F64) return convert_from<double>(pout, g, src, dst, nrhs, s);
+    if (pin == WL_F32) return convert_from<float>(pout, g, src, dst, nrhs, s);
+    if (pin == WL_F16) return convert_from<f16s>(pout, g, src, dst, nrhs, s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace wl

@@ -99,6 +99,9 @@ hipError_t launch_diff_norm2(const Geom &g, const void *b, const void *v, const 
 // dst = src for whole fields of nrhs right-hand sides
 hipError_t launch_copy(int prec, const Geom &g, const void *src, void *dst, int nrhs,
                        hipStream_t s);
+// dst (storage pout) = src (storage pin != pout), site by site, for nrhs right-hand sides
+hipError_t launch_convert(int pin, int pout, const Geom &g, const void *src, void *dst, int nrhs,
+                          hipStream_t s);
 
 // ---- multi-rhs (wl_multi.hip) ----
 // One launch of the register-blocked multi-rhs hopping kernel: right-hand sides [j0, j0+NB),

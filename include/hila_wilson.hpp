@@ -2,6 +2,8 @@
 // C ABI of libhila_wilson.so (wilson_mi355x.h).  Header-only.
 //
 //   Dirac_Wilson_evenodd<SU3d> D(kappa, U);     libraries/dirac/wilson.h:234-269
+//     ... D(kappa, U.fdir), D(D2), and typename decltype(D)::type_flt D_flt(D, U32)
+//     (the single-precision operator, wilson.h:250-253,267-269)
 //   D.apply(in, out);  D.dagger(in, out);       wilson.h:272-289   (out[ODD] = 0)
 //   D.kappa, D.par == EVEN, vector_type          wilson.h:241-256
 //   CG<Dirac_Wilson_evenodd<SU3d>> inv(D, accuracy, maxiters);   conjugate_gradient.h:33-40
@@ -14,6 +16,7 @@
 #pragma once
 
 #include <array>
+#include <atomic>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -39,6 +42,8 @@ inline int check(int rc, const char *what) {
     return rc;
 }
 
+class MatrixField;
+
 // The lattice geometry on one GPU (HILA's lattice.setup(size), lattice.h:477-480).
 class Lattice {
   public:
@@ -63,38 +68,110 @@ class Lattice {
         return v;
     }
 
+    // The device holds one gauge field per lattice.  upload_gauge puts the links of U there;
+    // bind_gauge does so only when the device does not already hold U as it is now (other
+    // fields, or fields changed since their upload), so that an operator holding a reference
+    // to U sees U's current links at every call, as HILA's operator reads its gauge fields.
+    inline void upload_gauge(const MatrixField (&U)[4]);
+    inline void bind_gauge(const MatrixField (&U)[4]);
+    // record that the device holds U as it is now (after a device-side load of U's links)
+    inline void note_bound(const MatrixField (&U)[4]);
+
   private:
     std::array<int, 4> size_;
     int64_t volume_ = 0;
     wl_ctx *ctx_ = nullptr;
+    const MatrixField *bound_[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t bound_gen_[4] = {0, 0, 0, 0};
 };
 
-// SU(3) gauge field: HILA's Field<SU<3,double>> gauge[4] (hmc/gauge_field.h:21), uploaded to
-// the GPU once; host layout V x 18 doubles per direction.
+// Field<SU<3,double>> (or Field<SquareMatrix<3,Complex<double>>>) of one direction on one
+// lattice: V x 18 doubles on the host in HILA's AoS layout (EVEN_SITES_FIRST, row-major 3x3
+// complex).  Every non-const access to the data counts as a change (a new generation), which
+// Lattice::bind_gauge compares with the generation it uploaded.
+class MatrixField {
+  public:
+    explicit MatrixField(Lattice &lat) : lat_(&lat), d_((size_t)18 * lat.volume()), gen_(next_generation()) {}
+    MatrixField(const MatrixField &o) : lat_(o.lat_), d_(o.d_), gen_(next_generation()) {}
+    MatrixField &operator=(const MatrixField &o) {
+        lat_ = o.lat_;
+        d_ = o.d_;
+        gen_ = next_generation();
+        return *this;
+    }
+    Lattice &lattice() const { return *lat_; }
+    std::vector<double> &data() {
+        gen_ = next_generation();
+        return d_;
+    }
+    const std::vector<double> &data() const { return d_; }
+    uint64_t generation() const { return gen_; }
+
+  private:
+    static uint64_t next_generation() {
+        static std::atomic<uint64_t> n{1};
+        return n++;
+    }
+    Lattice *lat_;
+    std::vector<double> d_;
+    uint64_t gen_;
+};
+
+inline void Lattice::note_bound(const MatrixField (&U)[4]) {
+    for (int mu = 0; mu < 4; ++mu) {
+        bound_[mu] = &U[mu];
+        bound_gen_[mu] = U[mu].generation();
+    }
+}
+inline void Lattice::upload_gauge(const MatrixField (&U)[4]) {
+    for (int mu = 0; mu < 4; ++mu)
+        if (&U[mu].lattice() != this) throw error(WL_E_ARG, "gauge field of another lattice");
+    const double *p[4] = {U[0].data().data(), U[1].data().data(), U[2].data().data(), U[3].data().data()};
+    check(wl_gauge_upload(ctx_, p), "wl_gauge_upload");
+    note_bound(U);
+}
+inline void Lattice::bind_gauge(const MatrixField (&U)[4]) {
+    for (int mu = 0; mu < 4; ++mu)
+        if (bound_[mu] != &U[mu] || bound_gen_[mu] != U[mu].generation()) {
+            upload_gauge(U);
+            return;
+        }
+}
+
+// SU(3) gauge field: HILA's GaugeField<SU<3,double>> with its direction fields fdir[4]
+// (libraries/plumbing/gaugefield.h:23-71; hmc/gauge_field.h:21), uploaded to the GPU.
 class GaugeField {
   public:
-    explicit GaugeField(Lattice &lat) : lat_(lat), U_(4, std::vector<double>(18 * lat.volume())) {}
-    std::vector<double> &direction(int mu) { return U_.at(mu); }
+    MatrixField fdir[4];
+
+    explicit GaugeField(Lattice &lat)
+        : fdir{MatrixField(lat), MatrixField(lat), MatrixField(lat), MatrixField(lat)}, lat_(lat) {}
+    std::vector<double> &direction(int mu) { return fdir[mu].data(); }
+    const std::vector<double> &direction(int mu) const { return fdir[mu].data(); }
     // kind 0 unit, 1 Haar random, 2 exp(i eps H)
     void random(int kind, double eps, uint64_t seed) {
-        double *p[4] = {U_[0].data(), U_[1].data(), U_[2].data(), U_[3].data()};
+        double *p[4] = {direction(0).data(), direction(1).data(), direction(2).data(), direction(3).data()};
         check(wl_gen_gauge(lat_.size().data(), kind, eps, seed, p), "wl_gen_gauge");
         upload();
     }
-    void upload() {
-        const double *p[4] = {U_[0].data(), U_[1].data(), U_[2].data(), U_[3].data()};
-        check(wl_gauge_upload(lat_.ctx(), p), "wl_gauge_upload");
-    }
+    void upload() { lat_.upload_gauge(fdir); }
+    // gauge_field_base::get_single_precision (hmc/gauge_field.h:83-90).  The library derives
+    // its FP32 copy of the device links itself on first use by a single-precision operator, so
+    // the single-precision gauge field is this one: Dirac_Wilson_evenodd<SU3f> built on it
+    // applies in FP32 with the FP32 copy of these links.
+    GaugeField &get_single_precision() { return *this; }
     // GaugeField::config_write / config_read (libraries/plumbing/gaugefield.h:194-272): HILA's
     // configuration file, read straight into the device gauge field (replicated when the file's
     // lattice divides this one); the host copy is refreshed from the device.
-    void config_write(const std::string &filename) const {
+    void config_write(const std::string &filename) {
+        lat_.bind_gauge(fdir);
         check(wl_config_write(lat_.ctx(), filename.c_str()), "wl_config_write");
     }
     void config_read(const std::string &filename) {
         check(wl_config_read(lat_.ctx(), filename.c_str()), "wl_config_read");
-        double *p[4] = {U_[0].data(), U_[1].data(), U_[2].data(), U_[3].data()};
+        double *p[4] = {direction(0).data(), direction(1).data(), direction(2).data(), direction(3).data()};
         check(wl_gauge_download(lat_.ctx(), p), "wl_gauge_download");
+        lat_.note_bound(fdir);
     }
     Lattice &lattice() { return lat_; }
 
@@ -118,26 +195,48 @@ class GaugeField {
 
   private:
     Lattice &lat_;
-    std::vector<std::vector<double>> U_;
 };
 
-// Field<WilsonVector<3,double>>: both parities on the GPU.
-class WilsonField {
+template <typename T>
+struct precision_of;
+template <>
+struct precision_of<double> {
+    static constexpr int value = WL_F64;
+};
+template <>
+struct precision_of<float> {
+    static constexpr int value = WL_F32;
+};
+
+// Field<WilsonVector<3,T>> (T = double or float): both parities on the GPU, stored in T.
+template <typename T = double>
+class WilsonFieldT {
   public:
-    explicit WilsonField(Lattice &lat) : lat_(&lat) {
-        check(wl_field_create(lat.ctx(), WL_EVEN, WL_F64, 1, &even_), "wl_field_create");
-        check(wl_field_create(lat.ctx(), WL_ODD, WL_F64, 1, &odd_), "wl_field_create");
+    using radix = T;
+    static constexpr int precision = precision_of<T>::value;
+
+    explicit WilsonFieldT(Lattice &lat) : lat_(&lat) {
+        check(wl_field_create(lat.ctx(), WL_EVEN, precision, 1, &even_), "wl_field_create");
+        check(wl_field_create(lat.ctx(), WL_ODD, precision, 1, &odd_), "wl_field_create");
     }
-    ~WilsonField() {
+    ~WilsonFieldT() {
         wl_field_destroy(even_);
         wl_field_destroy(odd_);
     }
-    WilsonField(const WilsonField &o) : WilsonField(*o.lat_) { *this = o; }
-    WilsonField &operator=(const WilsonField &o) {  // deep copy (field.h:898-909)
+    WilsonFieldT(const WilsonFieldT &o) : WilsonFieldT(*o.lat_) { *this = o; }
+    WilsonFieldT &operator=(const WilsonFieldT &o) {  // deep copy (field.h:898-909)
         if (this != &o) {
             check(wl_field_copy(even_, o.even_), "wl_field_copy");
             check(wl_field_copy(odd_, o.odd_), "wl_field_copy");
         }
+        return *this;
+    }
+    // assignment from a field of the other precision, converted on the device (HILA's
+    // c[ALL] = chi[X] between float and double fields, fermion_field.h:77-78,84)
+    template <typename U>
+    WilsonFieldT &operator=(const WilsonFieldT<U> &o) {
+        check(wl_field_copy(even_, o.part(EVEN)), "wl_field_copy");
+        check(wl_field_copy(odd_, o.part(ODD)), "wl_field_copy");
         return *this;
     }
     wl_field *part(Parity p) const { return p == ODD ? odd_ : even_; }
@@ -179,47 +278,69 @@ class WilsonField {
     Lattice *lat_;
     wl_field *even_ = nullptr, *odd_ = nullptr;
 };
+using WilsonField = WilsonFieldT<double>;
 
+// SU<3,double> and SU<3,float> as the operator's matrix type
 struct SU3d {
     static constexpr int size = 3;
+    using radix = double;
+};
+struct SU3f {
+    static constexpr int size = 3;
+    using radix = float;
 };
 
-// Dirac_Wilson_evenodd<SU<3,double>> (wilson.h:234-312); force() is not provided.
+// Dirac_Wilson_evenodd<SU<3,T>> (wilson.h:234-312).  It holds a non-owning reference to its
+// gauge fields, as HILA's does (wilson.h:237), and makes them the device's links at every
+// call (Lattice::bind_gauge).  force(chi, psi, force, sign) is wilson.h:294-311.
 template <typename matrix = SU3d>
 class Dirac_Wilson_evenodd {
   public:
     double kappa;
     static constexpr int N = matrix::size;
-    using vector_type = WilsonField;
+    using radix = typename matrix::radix;
+    using vector_type = WilsonFieldT<radix>;
     using matrix_type = matrix;
+    // The single-precision operator that preconditions the inversion of this one
+    // (wilson.h:250-253; fermion_action::initial_guess, fermion_field.h:73-84): it applies in
+    // FP32, on FP32 fields, with the library's FP32 copy of the same device links.
+    using type_flt = Dirac_Wilson_evenodd<SU3f>;
     Parity par = EVEN;
+    MatrixField (&gauge)[4];
 
-    Dirac_Wilson_evenodd(double k, GaugeField &U) : kappa(k), gauge_(U) {}
-    Dirac_Wilson_evenodd(Dirac_Wilson_evenodd &d) : kappa(d.kappa), gauge_(d.gauge_) {}
+    Dirac_Wilson_evenodd(Dirac_Wilson_evenodd &d) : kappa(d.kappa), gauge(d.gauge) {}  // wilson.h:259
+    Dirac_Wilson_evenodd(double k, MatrixField (&U)[4]) : kappa(k), gauge(U) {}         // :261
+    Dirac_Wilson_evenodd(double k, GaugeField &g) : kappa(k), gauge(g.fdir) {}          // :263-264
+    // from an operator of another matrix type, on the gauge field g (:267-269): e.g. the
+    // single-precision operator of a double one, on g.get_single_precision()
+    template <typename M>
+    Dirac_Wilson_evenodd(Dirac_Wilson_evenodd<M> &d, GaugeField &g) : kappa(d.kappa), gauge(g.fdir) {}
 
-    void apply(const WilsonField &in, WilsonField &out) { meo(in, out, +1); }
-    void dagger(const WilsonField &in, WilsonField &out) { meo(in, out, -1); }
+    void apply(const vector_type &in, vector_type &out) { meo(in, out, +1); }
+    void dagger(const vector_type &in, vector_type &out) { meo(in, out, -1); }
     // force(chi, psi, force, sign) (wilson.h:294-311): the derivative of Re <chi, M_sign psi>
     // with respect to the links, V x 18 doubles per direction
-    void force(const WilsonField &chi, const WilsonField &psi, GaugeField::LinkField &force, int sign) {
+    void force(const vector_type &chi, const vector_type &psi, GaugeField::LinkField &force, int sign) {
         double *p[4];
         for (int mu = 0; mu < 4; ++mu) {
-            force[mu].resize(18 * gauge_.lattice().volume());
+            force[mu].resize(18 * lattice().volume());
             p[mu] = force[mu].data();
         }
-        check(wl_fermion_force(gauge_.lattice().ctx(), chi.part(EVEN), psi.part(EVEN), kappa, sign, p),
-              "wl_fermion_force");
+        check(wl_fermion_force(bind(), chi.part(EVEN), psi.part(EVEN), kappa, sign, p), "wl_fermion_force");
     }
-    GaugeField &gauge() { return gauge_; }
+    Lattice &lattice() const { return gauge[0].lattice(); }
+    // the device context with this operator's gauge fields as its links
+    wl_ctx *bind() {
+        lattice().bind_gauge(gauge);
+        return lattice().ctx();
+    }
     double shift() const { return 0.0; }  // the operator is M itself
 
   private:
-    void meo(const WilsonField &in, WilsonField &out, int sign) {
-        check(wl_meo_apply(gauge_.lattice().ctx(), out.part(EVEN), in.part(EVEN), kappa, sign),
-              "wl_meo_apply");
+    void meo(const vector_type &in, vector_type &out, int sign) {
+        check(wl_meo_apply(bind(), out.part(EVEN), in.part(EVEN), kappa, sign), "wl_meo_apply");
         out.zero(ODD);  // out[ODD] = 0 (wilson.h:278,288)
     }
-    GaugeField &gauge_;
 };
 
 // Hasenbusch_operator<Dirac_type> (libraries/dirac/Hasenbusch.h:15-53): D + mh, whose apply /
@@ -228,6 +349,7 @@ template <typename Dirac_type>
 class Hasenbusch_operator {
   public:
     using vector_type = typename Dirac_type::vector_type;
+    using type_flt = Hasenbusch_operator<typename Dirac_type::type_flt>;  // Hasenbusch.h:19
     Dirac_type D;
     Parity par;
     double h_parameter;
@@ -237,19 +359,24 @@ class Hasenbusch_operator {
         : D(d), par(d.par), h_parameter(hasenbusch_parameter), kappa(d.kappa) {}
     Hasenbusch_operator(Hasenbusch_operator &h)
         : D(h.D), par(h.par), h_parameter(h.h_parameter), kappa(h.kappa) {}
+    // from a Hasenbusch operator of another Dirac type, on the gauge field mat
+    // (Hasenbusch.h:31-35): e.g. type_flt from the double operator
+    template <typename H_type, typename matrix>
+    Hasenbusch_operator(H_type &h, matrix &mat)
+        : D(h.D, mat), par(h.par), h_parameter(h.h_parameter), kappa(h.kappa) {}
 
     void apply(const vector_type &in, vector_type &out) { op(in, out, +1); }
     void dagger(const vector_type &in, vector_type &out) { op(in, out, -1); }
     void force(const vector_type &chi, const vector_type &psi, GaugeField::LinkField &f, int sign) {
         D.force(chi, psi, f, sign);  // Hasenbusch.h:50-53
     }
-    GaugeField &gauge() { return D.gauge(); }
+    Lattice &lattice() const { return D.lattice(); }
+    wl_ctx *bind() { return D.bind(); }
     double shift() const { return h_parameter; }
 
   private:
     void op(const vector_type &in, vector_type &out, int sign) {
-        check(wl_hasenbusch_apply(gauge().lattice().ctx(), out.part(EVEN), in.part(EVEN), kappa,
-                                  h_parameter, sign),
+        check(wl_hasenbusch_apply(bind(), out.part(EVEN), in.part(EVEN), kappa, h_parameter, sign),
               "wl_hasenbusch_apply");
         out.zero(ODD);
     }
@@ -261,7 +388,7 @@ class Hasenbusch_operator {
 template <typename DIRAC_OP>
 void MRE_guess(WilsonField &psi, const WilsonField &chi, DIRAC_OP &D,
                const std::vector<WilsonField> &old_chi_inv) {
-    wl_ctx *c = D.gauge().lattice().ctx();
+    wl_ctx *c = D.bind();
     const int n = (int)old_chi_inv.size();
     wl_field *basis = nullptr;
     check(wl_field_create(c, WL_EVEN, WL_F64, n > 0 ? n : 1, &basis), "wl_field_create");
@@ -284,8 +411,13 @@ class CG {
 
     // out: initial guess on entry, solution on exit.  Reaching maxiters is not an error (as in
     // HILA); the statistics of the last call are kept in `stats`.
+    //
+    // With a single-precision operator (Op::type_flt of a double one) the inner operator and
+    // the Krylov vectors are FP32 and the scalars FP64, as HILA's CG on float fields
+    // (conjugate_gradient.h:55-57); the solve runs with FP64 reliable updates on FP64 copies
+    // of in and out, and out receives the solution rounded to FP32.
     void apply(vector_type &in, vector_type &out) {
-        wl_ctx *c = M.gauge().lattice().ctx();
+        wl_ctx *c = M.bind();
         wl_cg_params p{};
         p.kappa = M.kappa;
         p.mh = M.shift();  // Hasenbusch_operator: (M + mh)^dag (M + mh)
@@ -294,7 +426,17 @@ class CG {
         p.inner_prec = inner_prec_;
         p.delta = delta_;
         p.update_every = update_every_;
-        const int rc = wl_cg_solve_ex(c, out.part(EVEN), in.part(EVEN), &p, &stats);
+        int rc;
+        if constexpr (vector_type::precision == WL_F64) {
+            rc = wl_cg_solve_ex(c, out.part(EVEN), in.part(EVEN), &p, &stats);
+        } else {
+            WilsonField in64(M.lattice()), out64(M.lattice());
+            in64 = in;
+            out64 = out;
+            p.inner_prec = vector_type::precision;
+            rc = wl_cg_solve_ex(c, out64.part(EVEN), in64.part(EVEN), &p, &stats);
+            if (rc == WL_OK || rc == WL_E_NOCONV) out = out64;
+        }
         if (rc != WL_E_NOCONV) check(rc, "wl_cg_solve_ex");
     }
     // Mixed precision (not in HILA's CG): inner operator and Krylov vectors in WL_F32 or WL_F16
@@ -340,12 +482,26 @@ class fermion_action {
         update_every_ = update_every;
     }
 
-    // initial_guess (:62-86): the MRE guess from the saved solutions (the FP32 pre-solve of
-    // HILA's double-precision builds is replaced by the reliable-update solver, set_solver)
+    // initial_guess (:62-86): the MRE guess from the saved solutions; then, when
+    // set_single_precision_guess(true), HILA's double-precision path (:73-84): a CG solve of
+    // the single-precision operator DIRAC_OP::type_flt, built by its converting constructor on
+    // gauge.get_single_precision(), from the guess, whose result becomes psi.  (By default the
+    // FP32 pre-solve is left to the reliable-update solver, set_solver.)
     void initial_guess(const WilsonField &c, WilsonField &psi) {
         psi.zero(ALL);
         if (MRE_size > 0 && !old_chi_inv.empty()) MRE_guess(psi, c, D, old_chi_inv);
+        if (single_precision_guess_) {
+            using D_flt_type = typename DIRAC_OP::type_flt;
+            D_flt_type D_flt(D, gauge.get_single_precision());
+            typename D_flt_type::vector_type cf(gauge.lattice()), pf(gauge.lattice());
+            cf = c;
+            pf = psi;
+            CG<D_flt_type> inverse(D_flt);
+            inverse.apply(cf, pf);
+            psi = pf;
+        }
     }
+    void set_single_precision_guess(bool on) { single_precision_guess_ = on; }
     // save_new_solution (:134-142): newest first, the oldest dropped
     void save_new_solution(const WilsonField &psi) {
         if (MRE_size <= 0) return;
@@ -374,7 +530,7 @@ class fermion_action {
         WilsonField psi(gauge.lattice());
         initial_guess(chi, psi);
         const wl_cg_params p = params();
-        const int rc = wl_fermion_force_step(gauge.lattice().ctx(), chi.part(EVEN), psi.part(EVEN), eps, &p, &stats);
+        const int rc = wl_fermion_force_step(D.bind(), chi.part(EVEN), psi.part(EVEN), eps, &p, &stats);
         if (rc != WL_E_NOCONV) check(rc, "wl_fermion_force_step");
         save_new_solution(psi);
     }
@@ -393,13 +549,14 @@ class fermion_action {
     }
     void solve(WilsonField &psi) {
         const wl_cg_params p = params();
-        const int rc = wl_cg_solve_ex(gauge.lattice().ctx(), psi.part(EVEN), chi.part(EVEN), &p, &stats);
+        const int rc = wl_cg_solve_ex(D.bind(), psi.part(EVEN), chi.part(EVEN), &p, &stats);
         if (rc != WL_E_NOCONV) check(rc, "wl_cg_solve_ex");
     }
     double accuracy_ = 1e-12;
     int inner_prec_ = WL_F64;
     double delta_ = 0.1;
     int update_every_ = 100;
+    bool single_precision_guess_ = false;
 };
 
 // Hasenbusch_action_1 (libraries/hmc/fermion_field.h:168-196): the fermion action of
@@ -478,7 +635,7 @@ class Hasenbusch_action_2 {
         p.accuracy = accuracy_;
         p.maxiters = 10000;
         p.inner_prec = WL_F64;
-        const int rc = wl_hasenbusch2_force_step(gauge.lattice().ctx(), chi.part(EVEN), psi.part(EVEN), eps, &p, &stats);
+        const int rc = wl_hasenbusch2_force_step(D.bind(), chi.part(EVEN), psi.part(EVEN), eps, &p, &stats);
         if (rc != WL_E_NOCONV) check(rc, "wl_hasenbusch2_force_step");
         save_new_solution(psi);
     }

@@ -142,8 +142,10 @@ int wl_field_destroy(wl_field *f);
 int wl_field_upload(wl_field *f, int rhs, const double *host);
 int wl_field_download(const wl_field *f, int rhs, double *host);
 int wl_field_zero(wl_field *f);
-/* dst = src (same parity, precision and nrhs): HILA's Field assignment dst[par] = src[X]
- * (libraries/plumbing/field.h:898-909), on the device. */
+/* dst = src (same parity and nrhs): HILA's Field assignment dst[par] = src[X]
+ * (libraries/plumbing/field.h:898-909), on the device.  When the precisions differ each
+ * component is converted (rounded once to the destination's precision), as HILA's assignment
+ * of a double field to a float one, c[ALL] = chi[X] (libraries/hmc/fermion_field.h:77-78). */
 int wl_field_copy(wl_field *dst, const wl_field *src);
 /* Right-hand side jd of dst = right-hand side js of src (same parity and precision): e.g.
  * keeping a solution in a slot of a basis of previous solutions, as

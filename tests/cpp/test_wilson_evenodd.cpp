@@ -7,6 +7,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <type_traits>
 #include <unistd.h>
 #include <vector>
 
@@ -312,11 +313,100 @@ static void run_hasenbusch_actions(Lattice &lattice) {
     CHECK(norm > 0 && ah < 1e-26 * norm, "Hasenbusch force steps: momentum change anti-Hermitian");
 }
 
+// The operator's constructors and its single-precision type (wilson.h:250-269):
+// D(kappa, Field<matrix> (&U)[NDIM]) gives the operator of D(kappa, GaugeField) bit for bit,
+// and sees a change of U at its next call (it holds a reference, wilson.h:237); type_flt built
+// by the converting constructor on gauge.get_single_precision() applies within FP32 rounding
+// of D and, used as fermion_action::initial_guess uses it (fermion_field.h:73-84: CG of the
+// FP32 operator on float copies, the result copied back to psi), leaves the FP64 CG fewer
+// iterations; fermion_action with that guess gives the same action.
+static void run_single_precision(Lattice &lattice) {
+    std::printf("Checking the operator constructors and type_flt\n");
+    GaugeField U(lattice);
+    U.random(2, 0.3, 51);
+    using dirac = Dirac_Wilson_evenodd<SU3d>;
+    static_assert(std::is_same<dirac::type_flt, Dirac_Wilson_evenodd<SU3f>>::value, "type_flt");
+    static_assert(std::is_same<dirac::type_flt::vector_type, WilsonFieldT<float>>::value, "float fields");
+    static_assert(std::is_same<Hasenbusch_operator<dirac>::type_flt, Hasenbusch_operator<dirac::type_flt>>::value,
+                  "Hasenbusch type_flt");
+    const double kappa = 0.124;
+    dirac D(kappa, U);
+    dirac Darr(kappa, U.fdir);
+    WilsonField a(lattice), Da(lattice), Da2(lattice);
+    a.gaussian_random(52, EVEN);
+    D.apply(a, Da);
+    Darr.apply(a, Da2);
+    CHECK(Da.host() == Da2.host(), "D(kappa, U.fdir) = D(kappa, U)");
+
+    // the operator holds the gauge fields by reference: a change of U shows at the next call
+    {
+        GaugeField V(lattice);
+        V.random(0, 0.0, 1);  // unit links
+        dirac Dv(kappa, V.fdir), Du(kappa, V);
+        WilsonField unit_out(lattice), changed_out(lattice);
+        Dv.apply(a, unit_out);
+        V.direction(2) = U.direction(2);  // replace the z links
+        Du.apply(a, changed_out);
+        CHECK(!(unit_out.host() == changed_out.host()), "operator sees its gauge field change");
+        V.direction(2).assign(V.direction(2).size(), 0.0);
+        for (int64_t x = 0; x < lattice.volume(); ++x)
+            for (int i = 0; i < 3; ++i) V.direction(2)[18 * x + 2 * 4 * i] = 1.0;  // unit again
+        Dv.apply(a, changed_out);
+        CHECK(unit_out.host() == changed_out.host(), "... and its restoration");
+    }
+
+    dirac::type_flt D_flt(D, U.get_single_precision());
+    CHECK(D_flt.kappa == D.kappa, "type_flt takes kappa from the double operator");
+    WilsonFieldT<float> af(lattice), Daf(lattice);
+    af = a;
+    D_flt.apply(af, Daf);
+    WilsonField back(lattice);
+    back = Daf;
+    CHECK(diff2_even(back, Da) < 1e-11 * Da.squarenorm(EVEN), "type_flt.apply within FP32 rounding of apply");
+    D_flt.dagger(af, Daf);
+    D.dagger(a, Da);
+    back = Daf;
+    CHECK(diff2_even(back, Da) < 1e-11 * Da.squarenorm(EVEN), "type_flt.dagger within FP32 rounding of dagger");
+
+    // initial_guess as fermion_field.h:73-84
+    WilsonField chi(lattice), psi(lattice), psi0(lattice);
+    chi.gaussian_random(53, EVEN);
+    chi.zero(ODD);
+    psi.zero(ALL);
+    WilsonFieldT<float> c(lattice), p(lattice);
+    c = chi;
+    p = psi;
+    CG<dirac::type_flt> inverse_flt(D_flt);
+    inverse_flt.apply(c, p);
+    psi = p;
+    CHECK(inverse_flt.stats.converged == 1, "CG of the single-precision operator");
+    CG<dirac> inverse(D, 1e-10);
+    inverse.apply(chi, psi);
+    const int it_guess = inverse.stats.iterations;
+    psi0.zero(ALL);
+    inverse.apply(chi, psi0);
+    const int it_zero = inverse.stats.iterations;
+    std::printf("      FP64 CG iterations: from zero %d, from the FP32 solve %d\n", it_zero, it_guess);
+    CHECK(it_guess * 2 <= it_zero && inverse.stats.true_resid <= 1e-10, "FP64 CG from the single-precision guess");
+    CHECK(diff2_even(psi, psi0) < 1e-16 * psi0.squarenorm(EVEN), "same solution from either start");
+
+    fermion_action<dirac> fa(D, U), fb(D, U);
+    fa.draw_gaussian_fields(54);
+    fb.draw_gaussian_fields(54);
+    fb.set_single_precision_guess(true);
+    const double sa = fa.action(), sb = fb.action();
+    CHECK(std::abs(sa - sb) < 1e-9 * std::abs(sa), "fermion_action with the single-precision guess");
+    Hasenbusch_operator<dirac> Dh(D, 0.1);
+    Hasenbusch_operator<dirac>::type_flt Dh_flt(Dh, U.get_single_precision());
+    CHECK(Dh_flt.h_parameter == 0.1 && Dh_flt.kappa == kappa, "Hasenbusch type_flt converting constructor");
+}
+
 int main() {
     try {
         Lattice lattice({16, 8, 8, 8});  // test_CG.cpp:57, NDIM == 4
         run(lattice, 0, "unit");
         run(lattice, 1, "Haar random");
+        run_single_precision(lattice);
         run_hasenbusch(lattice);
         run_force(lattice);
         run_hasenbusch_actions(lattice);
