@@ -113,16 +113,28 @@ def two_row_config(W, lat, D, fin, fodd, x64_full, n_solves):
 SMALL_DIMS, SMALL_KAPPA = (8, 8, 8, 8), 0.12
 
 
-def small_cg_config(W, n):
+def small_cg_config(W, n, dims=SMALL_DIMS, kind="haar", eps=0.0, gseed=1, kappa=SMALL_KAPPA, sseed=2):
     """The plain method's CPU-runnable configuration on the GPU: 8^4, Haar links (seed 1),
     kappa 0.12, Gaussian source (seed 2), FP64 CG to 1e-10.  3 MB of links and spinors sit in
-    the caches, so this measures launch latency, not bandwidth."""
-    U = W.gen_gauge(SMALL_DIMS, "haar", 0, 1)
-    lat = W.Lattice(SMALL_DIMS)
+    the caches, so this measures launch latency, not bandwidth.  Solved in one cooperative
+    launch per solve (the default at this size, WL_OPT_CG_PERSISTENT) and, for comparison,
+    kernel by kernel with a host check every 8 iterations; the two solutions are identical."""
+    U = W.gen_gauge(dims, kind, eps, gseed)
+    lat = W.Lattice(dims)
     lat.upload_gauge(U)
-    D = W.Dirac_Wilson_evenodd(SMALL_KAPPA, lat)
-    fin = W.Field(lat, W.EVEN).upload(W.gen_spinor(SMALL_DIMS, 2, W.EVEN))
-    out = solve_series(W, D, fin, W.Field(lat, W.EVEN), max(n, 5), "f64")
+    D = W.Dirac_Wilson_evenodd(kappa, lat)
+    fin = W.Field(lat, W.EVEN).upload(W.gen_spinor(dims, sseed, W.EVEN))
+    xp, xk = W.Field(lat, W.EVEN), W.Field(lat, W.EVEN)
+    lat.set_option(W.OPT_CG_PERSISTENT, 1)
+    out = solve_series(W, D, fin, xp, max(n, 5), "f64")
+    lat.set_option(W.OPT_CG_PERSISTENT, -1)
+    kbk = solve_series(W, D, fin, xk, max(n, 5), "f64")
+    lat.set_option(W.OPT_CG_PERSISTENT, 0)
+    out["lattice"] = list(dims)
+    out["mode"] = "one cooperative launch per solve"
+    out["kernel_by_kernel"] = kbk
+    out["speedup_vs_kernel_by_kernel"] = round(kbk["p50_ms"] / out["p50_ms"], 3)
+    out["solutions_bitwise_equal"] = bool(np.array_equal(xp.download(), xk.download()))
     lat.close()
     return out
 
@@ -708,6 +720,7 @@ def cpu_baseline(W, U, psi, seconds=8.0):
                         "unit": "GB/s", "ms_per_apply": round(1e3 * el1 / n1, 1),
                         "sample": f"{n1} applications, {el1:.1f} s, one thread"},
         "cg_8x8x8x8": small,
+        "cg_16x16x16x16": small16,
         "cg_32x32x32x32": big,
     }
 
@@ -844,6 +857,8 @@ def main(argv=None):
         print(f"# two-row links: {rows}", file=sys.stderr, flush=True)
     del x64, xmx
     small = small_cg_config(W, a.cg_solves) if a.cg_solves > 0 else None
+    small16 = (small_cg_config(W, a.cg_solves, (16, 16, 16, 16), "smooth", 0.3, 3, 0.125, 4)
+               if a.cg_solves > 0 else None)
     big = multi = serve = None
     D = fin = fodd = None
     lat.close()
@@ -892,6 +907,7 @@ def main(argv=None):
         "cg": cg_out,
         "cg_mixed_f32": cg_mixed,
         "cg_8x8x8x8": small,
+        "cg_16x16x16x16": small16,
         "two_row_links_32x32x32x32": rows,
         "mixed_48x48x48x96": big,
         "multi_rhs_32x32x32x32": multi,

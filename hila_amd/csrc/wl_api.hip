@@ -538,7 +538,7 @@ extern "C" int wl_destroy(wl_ctx *c) {
     c->user_fields.clear();
     for (void *p : {(void *)c->st_mr, (void *)c->flags_d, (void *)c->gauge64, c->gauge32, c->gauge16, c->rows64,
                     c->rows32, c->staging, (void *)c->partials, (void *)c->st, (void *)c->scratch_d, c->force64,
-                    c->mom64})
+                    c->mom64, c->grid_bar})
         if (p) (void)hipFree(p);
     if (c->st_host) (void)hipHostFree(c->st_host);
     if (c->scratch_h) (void)hipHostFree(c->scratch_h);
@@ -807,6 +807,11 @@ extern "C" int wl_set_option(wl_ctx *c, int option, int value) {
         if (value != 0 && value != 1) return fail(WL_E_ARG, "graphs option must be 0 or 1");
         c->use_graphs = value == 1;
         return WL_OK;
+    case WL_OPT_CG_PERSISTENT:
+        if (value < -1 || value > 1)
+            return fail(WL_E_ARG, "persistent CG option must be -1 (never), 0 (automatic) or 1 (always)");
+        c->cg_persistent = value;
+        return WL_OK;
     case WL_OPT_PIPELINE:
         if (value != 0 && value != 1) return fail(WL_E_ARG, "pipeline option must be 0 or 1");
         c->pipeline = value;
@@ -851,6 +856,7 @@ extern "C" int wl_get_option(const wl_ctx *c, int option, int *value) {
     case WL_OPT_GRAPHS: *value = c->use_graphs ? 1 : 0; return WL_OK;
     case WL_OPT_PIPELINE: *value = c->pipeline; return WL_OK;
     case WL_OPT_LINK_ROWS: *value = c->link_rows; return WL_OK;
+    case WL_OPT_CG_PERSISTENT: *value = c->cg_persistent; return WL_OK;
     default: return fail(WL_E_ARG, "unknown option %d", option);
     }
 }
@@ -859,12 +865,19 @@ extern "C" int wl_get_option(const wl_ctx *c, int option, int *value) {
 
 namespace {
 
+// A tuning holds for one lattice size, precision and rhs count, and for the link format and
+// link-load and pipelining modes it was timed with (they change the kernels' speed).
 struct TuneKey {
-    int L[4], prec, nrhs;
+    int L[4], prec, nrhs, link_format, link_load, pipeline;
     bool operator<(const TuneKey &o) const {
-        return std::tie(L[0], L[1], L[2], L[3], prec, nrhs) < std::tie(o.L[0], o.L[1], o.L[2], o.L[3], o.prec, o.nrhs);
+        return std::tie(L[0], L[1], L[2], L[3], prec, nrhs, link_format, link_load, pipeline) <
+               std::tie(o.L[0], o.L[1], o.L[2], o.L[3], o.prec, o.nrhs, o.link_format, o.link_load, o.pipeline);
     }
 };
+TuneKey tune_key(const wl_ctx *c, int prec, int nrhs) {
+    return TuneKey{{c->g.L[0], c->g.L[1], c->g.L[2], c->g.L[3]}, prec, nrhs, link_format(c, prec), c->link_load,
+                   c->pipeline};
+}
 std::mutex g_tune_mu;
 std::map<TuneKey, wl_tune_result> g_tuned;
 
@@ -888,8 +901,8 @@ int time_hop_mr(wl_ctx *c, int prec, void *out, const void *in, int N, int iters
 }  // namespace
 
 namespace wl {
-bool tuned_config(const Geom &g, int prec, int nrhs, wl_tune_result *r) {
-    const TuneKey key{{g.L[0], g.L[1], g.L[2], g.L[3]}, prec, nrhs};
+bool tuned_config(const wl_ctx *c, int prec, int nrhs, wl_tune_result *r) {
+    const TuneKey key = tune_key(c, prec, nrhs);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find(key);
     if (it == g_tuned.end()) return false;
@@ -921,7 +934,7 @@ extern "C" int wl_autotune(wl_ctx *c, int prec, int nrhs, wl_tune_result *out) {
     DeviceGuard dg(c->device);
     int rc = ensure_gauge(c, prec);
     if (rc) return rc;
-    const TuneKey key{{c->g.L[0], c->g.L[1], c->g.L[2], c->g.L[3]}, prec, nrhs};
+    const TuneKey key = tune_key(c, prec, nrhs);
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
         auto it = g_tuned.find(key);

@@ -6,8 +6,7 @@
 // finish kernel, in workgroup order, and the result goes straight into the device-resident CG
 // state.
 
-#include "wl_internal.h"
-#include "wl_store.h"
+#include "wl_blas.h"
 
 namespace wl {
 
@@ -147,12 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_norm2(const typename St<T>::E *__res
     if (done && *done) return;
     const int k = blockIdx.x * kBlock + threadIdx.x;
     double s = 0;
-    if (k < vh) {
-        cx<typename St<T>::R> w[4][3];
-        ld_site<T>(f + (size_t)j * St<T>::planes * S, S, k, w);
-#pragma unroll
-        for (int c = 0; c < 12; ++c) s += (double)norm2(w[c / 3][c % 3]);
-    }
+    if (k < vh) s = site_norm2<T>(f + (size_t)j * St<T>::planes * S, S, k);
     reduce_partial(s, ra);
 }
 
@@ -163,16 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(const cx<double> *__restrict
                                                     RedArgs ra) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     double s = 0;
-    if (k < vh) {
-#pragma unroll
-        for (int c = 0; c < 12; ++c) {
-            const int e = c * S + k;
-            const cx<double> v = b[e] - ddp[e];
-            r[e] = v;
-            p[e] = v;
-            s += norm2(v);
-        }
-    }
+    if (k < vh) s = site_cg_init(b, ddp, r, p, S, k);
     reduce_partial(s, ra);
 }
 
@@ -183,15 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_xpay(cx<R> *__restrict__ p,
     if (st->done) return;
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= vh) return;
-    const R beta = (R)st->beta;
-    cx<R> pv[4][3], rv[4][3];
-    ld_site<R>(p, S, k, pv);
-    ld_site<R>(r, S, k, rv);
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) pv[s][c] = beta * pv[s][c] + rv[s][c];
-    st_site<R>(p, S, k, pv);
+    site_xpay<R>(p, r, (R)st->beta, S, k);
 }
 
 // Reliable update: r = b - DDp in FP64 (DDp = M^dag M x in FP64), stored in the inner
@@ -222,13 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_norm2(const cx<double> *__restr
                                                        int S, RedArgs ra, int fin) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     double s = 0;
-    if (k < vh) {
-#pragma unroll
-        for (int c = 0; c < 12; ++c) {
-            const int e = c * S + k;
-            s += norm2(b[e] - v[e]);
-        }
-    }
+    if (k < vh) s = site_diff_norm2(b, v, S, k);
     reduce_partial(s, ra);
 }
 

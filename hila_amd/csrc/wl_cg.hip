@@ -20,6 +20,11 @@ namespace {
 
 int cg_f64(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int maxiters,
            wl_cg_stats *stats) {
+    if (cg_persistent_wanted(c)) {  // the same solve in one launch (wl_cgp.hip)
+        bool ran = false;
+        const int rc = cg_f64_persistent(c, x, b, kappa, accuracy, maxiters, stats, &ran);
+        if (ran || rc) return rc;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     int rc;
     for (wl_field **w : {&c->cg_r, &c->cg_p, &c->cg_Dp, &c->cg_DDp})

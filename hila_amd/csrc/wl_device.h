@@ -306,9 +306,10 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-// Checkerboard site index handled by this lane of a hopping-kernel launch (>= vh: idle lane).
-__device__ __forceinline__ int lane_site(const Geom &g) {
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+// Checkerboard site index handled by this lane as lane of workgroup b of a hopping-kernel
+// launch of nb workgroups (>= vh: idle lane).
+__device__ __forceinline__ int lane_site_of(const Geom &g, int b, int nb) {
+    const int tile = xcd_tile(b, nb);
     if (g.qc <= 0) return tile * blockDim.x + threadIdx.x;
     const int o = tile * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
     if (o >= g.ns) return g.vh;
@@ -317,6 +318,9 @@ __device__ __forceinline__ int lane_site(const Geom &g) {
     const int t = rem / g.qc, q = rem - t * g.qc;
     return ((t * g.sps + c * g.qc + q) << 6) + (int)(threadIdx.x & 63);
 }
+
+// ... of this launch's own workgroup
+__device__ __forceinline__ int lane_site(const Geom &g) { return lane_site_of(g, blockIdx.x, gridDim.x); }
 
 // ------------------------------------------------------------------- reductions ---------
 

@@ -63,6 +63,11 @@ inline void links_changed(wl_ctx *c) {
 }
 int ensure_gauge(wl_ctx *c, int prec);
 int check_sign(int sign);
+// The single-rhs FP64 CG in one cooperative launch (wl_cgp.hip), when WL_OPT_CG_PERSISTENT asks
+// for it; *ran = false, with nothing done, when the launch does not fit the device.
+bool cg_persistent_wanted(const wl_ctx *c);
+int cg_f64_persistent(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int maxiters,
+                      wl_cg_stats *stats, bool *ran);
 int ensure_partials(wl_ctx *c, size_t n);
 int ensure_st_mr(wl_ctx *c, int N);
 
@@ -91,9 +96,10 @@ int dslash_mr_all(wl_ctx *c, int prec, void *out, int out_par, const void *in, c
 int meo_mr(wl_ctx *c, int prec, void *out, const void *in, void *t, double kappa, int sign,
            int epi2, int N, const int *all_done, CgState *st, int *pstride_out);
 
-// The configuration wl_autotune chose for (lattice size, precision, nrhs) in this process, if
-// any: rhs per lane, rhs per launch, workgroup size.
-bool tuned_config(const Geom &g, int prec, int nrhs, wl_tune_result *r);
+// The configuration wl_autotune chose in this process for c's lattice size, link format and
+// link-load and pipelining modes, at (precision, nrhs), if any: rhs per lane, rhs per launch,
+// workgroup size.
+bool tuned_config(const wl_ctx *c, int prec, int nrhs, wl_tune_result *r);
 void apply_hop_config(wl_ctx *c, const wl_tune_result &r);   // rhs block/launch, block, LDS
 void set_lds_reserve(wl_ctx *c, int value);                  // WL_OPT_LDS_RESERVE semantics
 int lds_reserve_of(const wl_ctx *c);                         // ... and its current value

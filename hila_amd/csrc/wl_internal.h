@@ -223,6 +223,9 @@ struct wl_ctx {
     int link_load = 0;                 // multi-rhs links: 0 automatic, 1 cached, 2 nontemporal, 3 LDS
     bool use_graphs = false;           // batched CG: replay each chunk from a captured HIP graph
     int pipeline = 0;                  // multi-rhs hops: 0 pipelined where possible, 1 never
+    int cg_persistent = 0;             // single-rhs FP64 CG in one launch: -1 never, 0 small
+                                       // lattices, 1 always (where the launch fits)
+    void *grid_bar = nullptr;          // grid barrier of the persistent CG (on first use)
     bool blk_auto = true;              // g.blk is the volume's default (not chosen by a caller)
     bool lds_auto = true;              // g.lds unset: multi-rhs hops take mr_geom's occupancy cap
     double op_shift = 0.0;             // Hasenbusch shift mh of the operator being applied,

@@ -2,8 +2,9 @@
 //
 // Three host threads drive the device:
 //   loader     admits queued requests in submission order when a solve slot is free and the
-//              slot's device memory fits in the HBM budget, and enqueues the request's gauge
-//              field and sources on the slot's stream;
+//              slot's device memory fits in the HBM budget (a request that can start at once
+//              may pass a head that waits for memory, a bounded number of times), and enqueues
+//              the request's gauge field and sources on the slot's stream;
 //   scheduler  starts each admitted solve (BatchCG, batched over the request's right-hand
 //              sides) and pumps every running solve without waiting inside any of them, so
 //              the solves of all admitted requests overlap on the device, one stream each;
@@ -52,6 +53,7 @@ struct Request {
     wl_solve_result res{};
     std::vector<wl_cg_stats> stats;
     bool done = false;
+    int bypassed = 0;  // later requests admitted while this one waited at the head of the queue
 };
 
 struct Slot {
@@ -76,8 +78,11 @@ int slot_capacity(int n) {
     return n;
 }
 
-// Device bytes of a slot (fields at plane stride S, gauge copies, staging).
-size_t slot_estimate(const int dims[4], int prec, int cap) {
+// Device bytes of a slot (fields at plane stride S, gauge copies, staging).  link_rows: the
+// slots' WL_OPT_LINK_ROWS.  With two-row links BatchCG::prepare makes the FP64 two-row copy
+// for the reliable updates at every inner precision, and the FP32 inner operator reads an
+// FP32 two-row copy instead of the full FP32 links; FP16 always reads full FP16 links.
+size_t slot_estimate(const int dims[4], int prec, int cap, int link_rows) {
     const int64_t V = (int64_t)dims[0] * dims[1] * dims[2] * dims[3];
     const int64_t vh = V / 2;
     const int64_t S = (vh + kBlock - 1) / kBlock * kBlock;
@@ -86,11 +91,24 @@ size_t slot_estimate(const int dims[4], int prec, int cap) {
                                   + 4 * flo        // r, p, Dp (even), t (odd)
                                   + 3 * f64);      // FP64 update work
     if (prec != WL_F64) bytes += (size_t)cap * 12 * S * 8;  // FP32 inner solution
-    bytes += (size_t)72 * S * 16;                  // FP64 gauge
-    if (prec != WL_F64) bytes += (size_t)72 * S * elem_bytes(prec);
+    bytes += (size_t)72 * S * 16;                  // FP64 gauge (always held)
+    if (link_rows == 2) {
+        bytes += (size_t)48 * S * 16;              // FP64 two-row copy
+        if (prec == WL_F32) bytes += (size_t)48 * S * 8;   // FP32 two-row copy
+        if (prec == WL_F16) bytes += (size_t)72 * S * 4;   // FP16 links
+    } else if (prec != WL_F64) {
+        bytes += (size_t)72 * S * elem_bytes(prec);
+    }
     bytes += (size_t)V * 18 * 8;                   // staging (gauge upload)
     bytes += (size_t)cap * ((vh + 63) / 64) * 8 + (1 << 20);
     return bytes;
+}
+
+int slot_link_rows(const std::vector<std::pair<int, int>> &opts) {
+    int rows = 3;
+    for (const auto &o : opts)
+        if (o.first == WL_OPT_LINK_ROWS) rows = o.second;
+    return rows;
 }
 
 }  // namespace
@@ -221,26 +239,49 @@ bool slot_matches(const Slot &s, const wl_solve_request &r) {
            s.antiperiodic == (r.antiperiodic_t ? 1 : 0) && s.prec == r.inner_prec && s.cap >= r.nrhs;
 }
 
+// Admission: requests are taken in submission order.  When the head of the queue has to wait
+// for device memory (its slot does not fit the budget even after idle slots are released),
+// a later request among the next kBypassWindow may go first if it can start at once -- an
+// idle slot of its lattice is free, or its new slot fits the budget without releasing any --
+// until the head has been passed kMaxBypass times; then the queue waits for the head.
+constexpr int kBypassWindow = 8;
+constexpr int kMaxBypass = 4;
+
+// A slot for r that needs no memory beyond the budget: an idle slot of its lattice, or room
+// for a new one (*need > 0) without releasing anything.
+bool admissible_now(wl_server *S, const wl_solve_request &r, Slot **slot, size_t *need) {
+    *slot = nullptr;
+    *need = 0;
+    for (auto &sp : S->slots)
+        if (slot_matches(*sp, r) && (!*slot || sp->cap < (*slot)->cap)) *slot = sp.get();
+    if (*slot) return true;
+    *need = slot_estimate(r.dims, r.inner_prec, slot_capacity(r.nrhs), slot_link_rows(S->slot_opts));
+    return S->bytes_in_use + *need <= S->budget;
+}
+
 void loader_main(wl_server *S) {
     (void)hipSetDevice(S->device);
     std::unique_lock<std::mutex> lk(S->mu);
-    // generation at which the head of the queue did not fit; wait for a completion after it
+    // the queue waits for memory: for a completion after generation blocked_at, or for a new
+    // request that might be admitted ahead of the head
     uint64_t blocked_at = ~(uint64_t)0;
+    size_t blocked_qlen = 0;
     for (;;) {
         S->cv_load.wait(lk, [&] {
             return (S->stop && S->pending.empty()) ||
-                   (!S->pending.empty() && S->active < S->max_active && S->generation != blocked_at);
+                   (!S->pending.empty() && S->active < S->max_active &&
+                    (S->generation != blocked_at || S->pending.size() > blocked_qlen));
         });
         if (S->pending.empty()) return;  // stopping, queue drained
         std::shared_ptr<Request> q = S->pending.front();
-        const wl_solve_request &r = q->r;
         // an idle slot of this lattice (smallest sufficient capacity)
         Slot *slot = nullptr;
         for (auto &sp : S->slots)
-            if (slot_matches(*sp, r) && (!slot || sp->cap < slot->cap)) slot = sp.get();
+            if (slot_matches(*sp, q->r) && (!slot || sp->cap < slot->cap)) slot = sp.get();
+        size_t need = 0;
         if (!slot) {
-            const int cap = slot_capacity(r.nrhs);
-            const size_t need = slot_estimate(r.dims, r.inner_prec, cap);
+            const wl_solve_request &r = q->r;
+            need = slot_estimate(r.dims, r.inner_prec, slot_capacity(r.nrhs), slot_link_rows(S->slot_opts));
             // release idle slots, least recently used first, until the new one fits
             std::vector<std::unique_ptr<Slot>> victims;
             while (S->bytes_in_use + need > S->budget) {
@@ -269,18 +310,35 @@ void loader_main(wl_server *S) {
                              S->budget);
                     reject(S, q, WL_E_OOM, msg);
                     lk.lock();
-                } else {
-                    blocked_at = S->generation;  // wait for a request to finish
+                    continue;
                 }
-                continue;
+                // the head waits for memory: a later request that can start now goes first
+                std::shared_ptr<Request> pass;
+                if (q->bypassed < kMaxBypass) {
+                    const size_t n = std::min(S->pending.size(), (size_t)kBypassWindow + 1);
+                    for (size_t i = 1; i < n && !pass; ++i)
+                        if (admissible_now(S, S->pending[i]->r, &slot, &need)) pass = S->pending[i];
+                }
+                if (!pass) {
+                    blocked_at = S->generation;
+                    blocked_qlen = S->pending.size();
+                    continue;
+                }
+                q->bypassed += 1;
+                q = pass;
+            } else {
+                slot = nullptr;  // a new slot for the head
             }
+        }
+        const wl_solve_request &r = q->r;
+        if (!slot) {
             std::unique_ptr<Slot> sp;
             const std::vector<std::pair<int, int>> opts = S->slot_opts;
             lk.unlock();
-            int rc = create_slot(S, r, cap, opts, &sp);
+            int rc = create_slot(S, r, slot_capacity(r.nrhs), opts, &sp);
             lk.lock();
             if (rc) {
-                S->pending.pop_front();
+                S->pending.erase(std::find(S->pending.begin(), S->pending.end(), q));
                 const std::string msg = last_error();
                 lk.unlock();
                 reject(S, q, rc, msg.c_str());
@@ -293,7 +351,7 @@ void loader_main(wl_server *S) {
             S->slots_created += 1;
         }
         blocked_at = ~(uint64_t)0;
-        S->pending.pop_front();
+        S->pending.erase(std::find(S->pending.begin(), S->pending.end(), q));
         slot->busy = true;
         slot->req = q;
         S->active += 1;
@@ -332,7 +390,7 @@ void scheduler_main(wl_server *S) {
             // in this process, else the defaults (never timed here)
             wl_ctx *c = s->ctx;
             wl_tune_result tuned{};
-            if (tuned_config(c->g, r.inner_prec, r.nrhs, &tuned)) {
+            if (tuned_config(c, r.inner_prec, r.nrhs, &tuned)) {
                 apply_hop_config(c, tuned);
             } else {
                 c->rhs_block = 0;

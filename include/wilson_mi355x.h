@@ -318,7 +318,16 @@ int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, d
  *       1e-13 in every real part, else WL_E_ARG.  Results then agree with the full-link ones
  *       to rounding, not bit for bit.  Two-row copies take 2/3 of the gauge field's memory per
  *       precision used.  FP16 storage always reads full links.  The fermion force reads the
- *       full links. */
+ *       full links.
+ *   WL_OPT_CG_PERSISTENT  single-rhs FP64 CG (wl_cg_solve, wl_cg_solve_ex with FP64 inner):
+ *       1 = the whole solve -- source norm, initial residual, every iteration and the final
+ *       true residual -- runs as one cooperative launch whose workgroups meet at grid
+ *       barriers, with no host round trip inside the solve; -1 = kernel by kernel, with the
+ *       host checking convergence every 8 iterations; 0 (default) = one launch on lattices of
+ *       at most 2^15 sites per parity, where a solve is bound by launch latency.  Both give
+ *       the same solution, bit for bit: the persistent launch runs the same per-site code and
+ *       sums every reduction in the same order.  Falls back to kernel by kernel when the
+ *       launch does not fit on the device at once. */
 enum {
     WL_OPT_TRAVERSAL_CHUNK = 1,
     WL_OPT_BLOCK = 2,
@@ -329,7 +338,8 @@ enum {
     WL_OPT_LDS_RESERVE = 7,
     WL_OPT_GRAPHS = 8,
     WL_OPT_PIPELINE = 9,
-    WL_OPT_LINK_ROWS = 10
+    WL_OPT_LINK_ROWS = 10,
+    WL_OPT_CG_PERSISTENT = 11
 };
 int wl_set_option(wl_ctx *ctx, int option, int value);
 int wl_get_option(const wl_ctx *ctx, int option, int *value);
@@ -342,9 +352,10 @@ int wl_get_option(const wl_ctx *ctx, int option, int *value);
  * FP32 also WL_OPT_LDS_RESERVE: automatic, i.e. the occupancy cap where it applies, or none).
  * The choice is timed again against the default configuration, which is kept unless the choice
  * is faster.  The choice is
- * remembered per (lattice size, precision, nrhs) for the life of the process, so that later
- * calls, and other contexts of the same size, take the same configuration without timing
- * again (results then do not depend on timing noise).  The gauge field must be uploaded. */
+ * remembered per (lattice size, precision, nrhs, link format WL_OPT_LINK_ROWS, WL_OPT_LINK_LOAD,
+ * WL_OPT_PIPELINE) for the life of the process, so that later calls, and other contexts of the
+ * same size and options, take the same configuration without timing again (results then do
+ * not depend on timing noise).  The gauge field must be uploaded. */
 typedef struct wl_tune_result {
     int rhs_block;            /* 0 in all three: the default configuration was fastest      */
     int rhs_launch;
@@ -380,8 +391,12 @@ int wl_time_op(wl_ctx *ctx, int what, wl_field *out, const wl_field *in, double 
  * context with the gauge field, the rhs/solution fields and the CG work fields of a lattice
  * size) stay resident after a request and are reused by the next request of that size; idle
  * slots of other sizes are released when the budget requires it.  Requests are admitted in
- * submission order.  The solves of admitted requests overlap on the device: each slot has its
- * own stream, and the host drives all of them without waiting inside a solve. */
+ * submission order, except that when the oldest waiting request needs memory that running
+ * requests hold, one of the next 8 that can start at once (an idle slot of its lattice, or
+ * room for a new slot without releasing any) goes first; the oldest is passed at most 4
+ * times, then the queue waits for it.  The solves of admitted requests overlap on the device:
+ * each slot has its own stream, and the host drives all of them without waiting inside a
+ * solve. */
 typedef struct wl_server wl_server;
 
 typedef struct wl_solve_request {

@@ -127,6 +127,24 @@ def test_links_that_are_not_special_unitary_are_refused():
     lat.close()
 
 
+@pytest.mark.parametrize("where,value", [((8, 0), np.nan), ((7, 1), np.inf), ((0, 0), np.nan)],
+                         ids=["nan_row2", "inf_row2", "nan_row0"])
+def test_links_with_nan_or_inf_entries_are_refused(where, value):
+    """A NaN or infinite entry in any row makes the stored third row fail the check: the
+    full-link operator would return NaN there, so two-row links must not replace it."""
+    dims = (4, 4, 4, 4)
+    U = W.gen_gauge(dims, "haar", 0, 36)
+    entry, part = where
+    U[1, 5, 2 * entry + part] = value
+    lat = W.Lattice(dims)
+    lat.upload_gauge(U)
+    with pytest.raises(W.WilsonError) as e:
+        lat.set_option(W.OPT_LINK_ROWS, 2)
+    assert e.value.code == W.WL_E_ARG
+    assert lat.get_option(W.OPT_LINK_ROWS) == 3
+    lat.close()
+
+
 def test_a_new_upload_replaces_the_two_row_copies():
     dims = (8, 8, 8, 8)
     lat, U1 = _lattice(dims, "haar", 38, 2)
