@@ -112,6 +112,14 @@ RedArgs red_args(wl_ctx *c) {
     return ra;
 }
 
+RedArgs red_args(const CgWork &w) {
+    RedArgs ra;
+    ra.partials = w.partials;
+    ra.st = w.st;
+    ra.out = w.red_out;
+    return ra;
+}
+
 // Stored entries per link of the links the hopping kernels of precision prec read: 6 for
 // two-row links (WL_OPT_LINK_ROWS = 2, FP64 and FP32), else 9.
 int link_format(const wl_ctx *c, int prec) { return c->link_rows == 2 && prec != WL_F16 ? 6 : 9; }
@@ -189,6 +197,11 @@ int ensure_gauge(wl_ctx *c, int prec) {
 // out_par = a * H_sign in (+ xin), one rhs
 int dslash_one(wl_ctx *c, int prec, void *out, int out_par, const void *in, const void *xin,
                double a, int sign, int epi, const int *done, int fin) {
+    return dslash_one(c, red_args(c), prec, out, out_par, in, xin, a, sign, epi, done, fin);
+}
+
+int dslash_one(wl_ctx *c, const RedArgs &ra, int prec, void *out, int out_par, const void *in, const void *xin,
+               double a, int sign, int epi, const int *done, int fin) {
     DslashLaunch L{};
     L.out = out;
     L.in = in;
@@ -202,7 +215,7 @@ int dslash_one(wl_ctx *c, int prec, void *out, int out_par, const void *in, cons
     L.sign = sign;
     L.epi = epi;
     L.done = done;
-    L.red = red_args(c);
+    L.red = ra;
     L.fin = fin;
     HIPCHK(launch_dslash(prec, c->g, L, c->stream));
     return WL_OK;
@@ -211,11 +224,16 @@ int dslash_one(wl_ctx *c, int prec, void *out, int out_par, const void *in, cons
 // M_s in -> out for even FP64/FP32 fields (one rhs), through the odd temporary tmp:
 // tmp_o = -kappa H_oe in  (Dirac_Wilson_hop_set(..., ODD, sign), wilson.h:275,285)
 // out_e = in + kappa H_eo tmp_o  (Dirac_Wilson_hop(-kappa, out, out, EVEN, sign), :277,287)
+int meo_one(wl_ctx *c, const RedArgs &ra, int prec, void *out, const void *in, void *tmp, double kappa,
+            int sign, int epi2, const int *done, int fin2) {
+    int rc = dslash_one(c, ra, prec, tmp, 1, in, nullptr, -kappa, sign, EPI_SET, done, FIN_NONE);
+    if (rc) return rc;
+    return dslash_one(c, ra, prec, out, 0, tmp, in, kappa, sign, epi2, done, fin2);
+}
+
 int meo_one(wl_ctx *c, int prec, void *out, const void *in, void *tmp, double kappa, int sign,
             int epi2, const int *done, int fin2) {
-    int rc = dslash_one(c, prec, tmp, 1, in, nullptr, -kappa, sign, EPI_SET, done, FIN_NONE);
-    if (rc) return rc;
-    return dslash_one(c, prec, out, 0, tmp, in, kappa, sign, epi2, done, fin2);
+    return meo_one(c, red_args(c), prec, out, in, tmp, kappa, sign, epi2, done, fin2);
 }
 
 int check_sign(int sign) { return (sign == 1 || sign == -1) ? WL_OK : fail(WL_E_ARG, "sign must be +-1"); }

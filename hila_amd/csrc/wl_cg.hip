@@ -18,39 +18,31 @@ namespace wl {
 
 namespace {
 
-int cg_f64(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int maxiters,
-           wl_cg_stats *stats) {
-    if (cg_persistent_wanted(c)) {  // the same solve in one launch (wl_cgp.hip)
-        bool ran = false;
-        const int rc = cg_f64_persistent(c, x, b, kappa, accuracy, maxiters, stats, &ran);
-        if (ran || rc) return rc;
-    }
+// The kernel-by-kernel FP64 CG on work memory w.
+int cg_f64(wl_ctx *c, const CgWork &w, void *x, size_t x_bytes, const void *b, double kappa, double accuracy,
+           int maxiters, wl_cg_stats *stats) {
     const auto t0 = std::chrono::steady_clock::now();
     int rc;
-    for (wl_field **w : {&c->cg_r, &c->cg_p, &c->cg_Dp, &c->cg_DDp})
-        if ((rc = ensure_tmp(c, WL_F64, w, WL_EVEN))) return rc;
-    if ((rc = ensure_tmp(c, WL_F64, &c->tmp_odd64, WL_ODD))) return rc;
-    void *r = c->cg_r->data, *p = c->cg_p->data, *Dp = c->cg_Dp->data, *DDp = c->cg_DDp->data;
-    void *t = c->tmp_odd64->data;
-    const RedArgs ra = red_args(c);
-    const int *done = &c->st->done;
+    void *r = w.r, *p = w.p, *Dp = w.Dp, *DDp = w.DDp, *t = w.t;
+    const RedArgs ra = red_args(w);
+    const int *done = &w.st->done;
 
     CgState init{};
     init.acc2 = accuracy * accuracy;
     init.maxiters = maxiters;
     *c->st_host = init;
-    HIPCHK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(w.st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
     // source_norm and target_rr (conjugate_gradient.h:61-63)
     HIPCHK(launch_norm2(WL_F64, c->g, b, 1, 0, ra, FIN_SOURCE, nullptr, c->stream));
     // r = b - M^dag M x ; p = r ; rr = |r|^2  (:65-73)
-    if ((rc = meo_one(c, WL_F64, Dp, x, t, kappa, +1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
-    if ((rc = meo_one(c, WL_F64, DDp, Dp, t, kappa, -1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
+    if ((rc = meo_one(c, ra, WL_F64, Dp, x, t, kappa, +1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
+    if ((rc = meo_one(c, ra, WL_F64, DDp, Dp, t, kappa, -1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
     HIPCHK(launch_cg_init(c->g, b, DDp, r, p, ra, c->stream));
-    HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->st_host, w.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->st_host->source_norm == 0.0) {
         // b = 0: the solution is x = 0 (HILA would divide 0 by 0 here)
-        HIPCHK(hipMemsetAsync(x, 0, c->cg_r->bytes, c->stream));
+        HIPCHK(hipMemsetAsync(x, 0, x_bytes, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));
@@ -60,23 +52,21 @@ int cg_f64(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int
     }
     if (maxiters <= 0) {
         c->st_host->done = 2;
-        HIPCHK(hipMemcpyAsync(&c->st->done, &c->st_host->done, sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(&w.st->done, &c->st_host->done, sizeof(int), hipMemcpyHostToDevice, c->stream));
     }
 
     // main loop (:75-96), kChunk iterations per enqueue; the kernels of an iteration after
     // convergence return at once.
     int enqueued = 0;
-    hipEvent_t ev;
-    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     int done_h = c->st_host->done;
     while (!done_h && enqueued < maxiters) {
         const int n = std::min(kChunk, maxiters - enqueued);
         for (int it = 0; it < n; ++it) {
             // Dp = M p, pDp = |Dp|^2, alpha = rr / pDp
-            if ((rc = meo_one(c, WL_F64, Dp, p, t, kappa, +1, EPI_AXPY_NORM, done, FIN_ALPHA)))
+            if ((rc = meo_one(c, ra, WL_F64, Dp, p, t, kappa, +1, EPI_AXPY_NORM, done, FIN_ALPHA)))
                 return rc;
             // DDp = M^dag Dp fused with x += alpha p, r -= alpha DDp, rrnew = |r|^2, test, beta
-            if ((rc = dslash_one(c, WL_F64, t, 1, Dp, nullptr, -kappa, -1, EPI_SET, done, FIN_NONE)))
+            if ((rc = dslash_one(c, ra, WL_F64, t, 1, Dp, nullptr, -kappa, -1, EPI_SET, done, FIN_NONE)))
                 return rc;
             DslashLaunch L{};
             L.out = nullptr;
@@ -98,21 +88,19 @@ int cg_f64(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int
             L.fin = FIN_RRNEW;
             HIPCHK(launch_dslash(WL_F64, c->g, L, c->stream));
             // p = beta p + r
-            HIPCHK(launch_cg_xpay(WL_F64, c->g, p, r, c->st, c->stream));
+            HIPCHK(launch_cg_xpay(WL_F64, c->g, p, r, w.st, c->stream));
         }
         enqueued += n;
-        HIPCHK(hipMemcpyAsync(&c->st_host->done, &c->st->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipEventRecord(ev, c->stream));
-        HIPCHK(hipEventSynchronize(ev));
+        HIPCHK(hipMemcpyAsync(&c->st_host->done, &w.st->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
         done_h = c->st_host->done;
     }
-    (void)hipEventDestroy(ev);
 
     // true residual |b - M^dag M x|^2 recomputed in FP64
-    if ((rc = meo_one(c, WL_F64, Dp, x, t, kappa, +1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
-    if ((rc = meo_one(c, WL_F64, DDp, Dp, t, kappa, -1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
+    if ((rc = meo_one(c, ra, WL_F64, Dp, x, t, kappa, +1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
+    if ((rc = meo_one(c, ra, WL_F64, DDp, Dp, t, kappa, -1, EPI_AXPY, nullptr, FIN_NONE))) return rc;
     HIPCHK(launch_diff_norm2(c->g, b, DDp, ra, FIN_TRUE, c->stream));
-    HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->st_host, w.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     const CgState &S = *c->st_host;
     const bool conv = S.done == 1;
@@ -566,6 +554,43 @@ int cg_batched(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec, wl_
 }
 
 }  // namespace
+
+size_t cg_partials_needed(const wl_ctx *c) {
+    return (size_t)std::max((c->g.vh + c->g.blk - 1) / c->g.blk, nblocks(c->g));
+}
+
+int cg_work_of(wl_ctx *c, CgWork *w) {
+    int rc;
+    for (wl_field **f : {&c->cg_r, &c->cg_p, &c->cg_Dp, &c->cg_DDp})
+        if ((rc = ensure_tmp(c, WL_F64, f, WL_EVEN))) return rc;
+    if ((rc = ensure_tmp(c, WL_F64, &c->tmp_odd64, WL_ODD))) return rc;
+    if ((rc = ensure_partials(c, cg_partials_needed(c)))) return rc;
+    if (!c->grid_bar)
+        if ((rc = dev_alloc(c, &c->grid_bar, 256, "grid barrier"))) return rc;
+    w->r = c->cg_r->data;
+    w->p = c->cg_p->data;
+    w->Dp = c->cg_Dp->data;
+    w->DDp = c->cg_DDp->data;
+    w->t = c->tmp_odd64->data;
+    w->partials = c->partials;
+    w->partials_cap = c->partials_cap;
+    w->st = c->st;
+    w->bar = c->grid_bar;
+    w->red_out = c->scratch_d;
+    return WL_OK;
+}
+
+int cg_f64_run(wl_ctx *c, const CgWork &w, void *x, size_t x_bytes, const void *b, double kappa,
+               double accuracy, int maxiters, wl_cg_stats *stats) {
+    if (w.partials_cap < cg_partials_needed(c)) return fail(WL_E_STATE, "CG work memory: too few partial sums");
+    if (cg_persistent_wanted(c)) {  // the same solve in one launch (wl_cgp.hip)
+        bool ran = false;
+        const int rc = cg_f64_persistent(c, w, x, x_bytes, b, kappa, accuracy, maxiters, stats, &ran);
+        if (ran || rc) return rc;
+    }
+    return cg_f64(c, w, x, x_bytes, b, kappa, accuracy, maxiters, stats);
+}
+
 }  // namespace wl
 
 using namespace wl;
@@ -595,7 +620,11 @@ int cg_solve(wl_ctx *c, wl_field *x, const wl_field *b, const wl_cg_params &p, b
     OpShift os(c, p.mh);
     int rc = ensure_gauge(c, WL_F64);
     if (rc) return rc;
-    if (!mixed && b->nrhs == 1) return cg_f64(c, x->data, b->data, p.kappa, p.accuracy, p.maxiters, stats);
+    if (!mixed && b->nrhs == 1) {
+        CgWork w;
+        if ((rc = cg_work_of(c, &w))) return rc;
+        return cg_f64_run(c, w, x->data, x->bytes, b->data, p.kappa, p.accuracy, p.maxiters, stats);
+    }
     if (mixed && b->nrhs == 1 && p.inner_prec != WL_F16)  // FP16 inner: the batched solver, any count
         return cg_mixed(c, x->data, b->data, p.kappa, p.accuracy, p.maxiters, p.inner_prec, p.delta,
                         p.update_every, stats);

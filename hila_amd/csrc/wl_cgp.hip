@@ -265,23 +265,27 @@ bool cg_persistent_wanted(const wl_ctx *c) {
     return c->cg_persistent > 0 || (c->cg_persistent == 0 && c->g.vh <= kPersistentAutoVh);
 }
 
-// The persistent FP64 CG; *ran = false (and nothing done) when the launch does not fit the
-// device, so that the caller solves kernel by kernel.
-int cg_f64_persistent(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int maxiters,
-                      wl_cg_stats *stats, bool *ran) {
+__global__ void k_zero_words(unsigned *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_zero_words(void *p, int words, hipStream_t s) {
+    if (words <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, (unsigned *)p, words);
+    return hipGetLastError();
+}
+
+// The persistent FP64 CG on work memory w; *ran = false (and nothing done) when the launch does
+// not fit the device, so that the caller solves kernel by kernel.
+int cg_f64_persistent(wl_ctx *c, const CgWork &w, void *x, size_t x_bytes, const void *b, double kappa,
+                      double accuracy, int maxiters, wl_cg_stats *stats, bool *ran) {
     *ran = false;
     const auto t0 = std::chrono::steady_clock::now();
     const int lf = link_format(c, WL_F64);
     const int G = lf == 6 ? persistent_grid<6>(c) : persistent_grid<9>(c);
     if (G <= 0) return WL_OK;
-    int rc;
-    for (wl_field **w : {&c->cg_r, &c->cg_p, &c->cg_Dp, &c->cg_DDp})
-        if ((rc = ensure_tmp(c, WL_F64, w, WL_EVEN))) return rc;
-    if ((rc = ensure_tmp(c, WL_F64, &c->tmp_odd64, WL_ODD))) return rc;
     const int nbh = (c->g.vh + c->g.blk - 1) / c->g.blk;
-    if ((rc = ensure_partials(c, (size_t)std::max(nbh, nblocks(c->g))))) return rc;
-    if (!c->grid_bar)
-        if ((rc = dev_alloc(c, &c->grid_bar, sizeof(GridBar), "grid barrier"))) return rc;
     CgpArgs P;
     P.g = c->g;
     P.gauge = (const cx<double> *)gauge_for(c, WL_F64);
@@ -290,14 +294,14 @@ int cg_f64_persistent(wl_ctx *c, void *x, const void *b, double kappa, double ac
     P.bshift = 1.0 + c->op_shift;
     P.x = (cx<double> *)x;
     P.b = (const cx<double> *)b;
-    P.r = (cx<double> *)c->cg_r->data;
-    P.p = (cx<double> *)c->cg_p->data;
-    P.Dp = (cx<double> *)c->cg_Dp->data;
-    P.DDp = (cx<double> *)c->cg_DDp->data;
-    P.t = (cx<double> *)c->tmp_odd64->data;
-    P.partials = c->partials;
-    P.st = c->st;
-    P.bar = (GridBar *)c->grid_bar;
+    P.r = (cx<double> *)w.r;
+    P.p = (cx<double> *)w.p;
+    P.Dp = (cx<double> *)w.Dp;
+    P.DDp = (cx<double> *)w.DDp;
+    P.t = (cx<double> *)w.t;
+    P.partials = w.partials;
+    P.st = w.st;
+    P.bar = (GridBar *)w.bar;
     P.nbh = nbh;
     P.nb256 = nblocks(c->g);
 
@@ -305,8 +309,8 @@ int cg_f64_persistent(wl_ctx *c, void *x, const void *b, double kappa, double ac
     init.acc2 = accuracy * accuracy;
     init.maxiters = maxiters;
     *c->st_host = init;
-    HIPCHK(hipMemcpyAsync(c->st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemsetAsync(c->grid_bar, 0, sizeof(GridBar), c->stream));
+    HIPCHK(hipMemcpyAsync(w.st, c->st_host, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_zero_words(w.bar, (int)(sizeof(GridBar) / 4), c->stream));
     void *args[] = {&P};
     const void *fn = lf == 6 ? (const void *)k_cg_persistent<6> : (const void *)k_cg_persistent<9>;
     const hipError_t le = hipLaunchCooperativeKernel(fn, dim3(G), dim3(c->g.blk), args, 0, c->stream);
@@ -316,8 +320,8 @@ int cg_f64_persistent(wl_ctx *c, void *x, const void *b, double kappa, double ac
     }
     HIPCHK(le);
     *ran = true;
-    HIPCHK(hipMemcpyAsync(c->st_host, c->st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(c->scratch_h, c->grid_bar, sizeof(GridBar), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->st_host, w.st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->scratch_h, w.bar, sizeof(GridBar), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     GridBar bar;
     std::memcpy(&bar, c->scratch_h, sizeof(bar));
@@ -325,7 +329,7 @@ int cg_f64_persistent(wl_ctx *c, void *x, const void *b, double kappa, double ac
     const CgState &S = *c->st_host;
     if (S.source_norm == 0.0) {
         // b = 0: the solution is x = 0 (HILA would divide 0 by 0 here)
-        HIPCHK(hipMemsetAsync(x, 0, c->cg_r->bytes, c->stream));
+        HIPCHK(hipMemsetAsync(x, 0, x_bytes, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         if (stats) {
             std::memset(stats, 0, sizeof(*stats));

@@ -63,20 +63,50 @@ inline void links_changed(wl_ctx *c) {
 }
 int ensure_gauge(wl_ctx *c, int prec);
 int check_sign(int sign);
+
+// Device work memory of the single-rhs FP64 CG: the even fields r, p, Dp, DDp, the odd
+// temporary t of M, the reduction partials, the CG state, the grid barrier of the one-launch
+// solve and the target of stored reductions.  Either the context's own work fields (created
+// on first use, cg_work_of) or a fixed workspace.
+struct CgWork {
+    void *r, *p, *Dp, *DDp, *t;
+    double *partials;
+    size_t partials_cap;  // doubles
+    CgState *st;
+    void *bar;            // GridBar of the one-launch solve
+    double *red_out;
+};
+int cg_work_of(wl_ctx *c, CgWork *w);
+// Partial sums one pass of the single-rhs CG writes at most: one per hopping workgroup or
+// per 256-site BLAS workgroup.
+size_t cg_partials_needed(const wl_ctx *c);
+RedArgs red_args(const CgWork &w);
+// The single-rhs FP64 CG (conjugate_gradient.h:46-104) on work memory w: in one cooperative
+// launch when WL_OPT_CG_PERSISTENT asks for it and the launch fits, else kernel by kernel.
+// x_bytes: bytes of the solution field (cleared for b = 0).  Makes no device allocation.
+int cg_f64_run(wl_ctx *c, const CgWork &w, void *x, size_t x_bytes, const void *b, double kappa,
+               double accuracy, int maxiters, wl_cg_stats *stats);
 // The single-rhs FP64 CG in one cooperative launch (wl_cgp.hip), when WL_OPT_CG_PERSISTENT asks
 // for it; *ran = false, with nothing done, when the launch does not fit the device.
 bool cg_persistent_wanted(const wl_ctx *c);
-int cg_f64_persistent(wl_ctx *c, void *x, const void *b, double kappa, double accuracy, int maxiters,
-                      wl_cg_stats *stats, bool *ran);
+int cg_f64_persistent(wl_ctx *c, const CgWork &w, void *x, size_t x_bytes, const void *b, double kappa,
+                      double accuracy, int maxiters, wl_cg_stats *stats, bool *ran);
+// words 32-bit words at p = 0, by a kernel (no memset node if a caller captures the stream)
+hipError_t launch_zero_words(void *p, int words, hipStream_t s);
 int ensure_partials(wl_ctx *c, size_t n);
 int ensure_st_mr(wl_ctx *c, int N);
 
-// out_par = a * H_sign in (+ xin), one rhs
+// out_par = a * H_sign in (+ xin), one rhs; reductions through the context's partials, or
+// through ra
 int dslash_one(wl_ctx *c, int prec, void *out, int out_par, const void *in, const void *xin,
+               double a, int sign, int epi, const int *done, int fin);
+int dslash_one(wl_ctx *c, const RedArgs &ra, int prec, void *out, int out_par, const void *in, const void *xin,
                double a, int sign, int epi, const int *done, int fin);
 // M_s in -> out (one rhs) through the odd temporary tmp
 int meo_one(wl_ctx *c, int prec, void *out, const void *in, void *tmp, double kappa, int sign,
             int epi2, const int *done, int fin2);
+int meo_one(wl_ctx *c, const RedArgs &ra, int prec, void *out, const void *in, void *tmp, double kappa,
+            int sign, int epi2, const int *done, int fin2);
 
 // Work fields of the solvers, one per (slot, precision, parity), created on first use with at
 // least the requested number of right-hand sides and kept resident with the context.
