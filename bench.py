@@ -665,7 +665,9 @@ def cpu_baseline(W, U, psi, seconds=8.0):
     """The oracle's HILA-structured hopping term (projection pass + per-direction
     accumulation, wilson.h:53-95) on the host cores, on a bounded number of applications;
     then the same on one core, and the oracle's CG (conjugate_gradient.h:46-104) to 1e-10 at
-    8^4 (Haar seed 1, kappa 0.12, source seed 2) and at 32^4 (the Dslash's field and source)."""
+    8^4 (Haar seed 1, kappa 0.12, source seed 2), at 16^4 (links exp(i 0.3 H) seed 3, kappa
+    0.125, source seed 4: the GPU's 16^4 CG configuration) and at 32^4 (the Dslash's field and
+    source)."""
     from oracle import oracle as O
     orc = O.Oracle(DIMS)
     orc.hop_set(U, KAPPA, psi, O.ODD, 1)  # warm-up (page-in, thread pool)
@@ -699,6 +701,8 @@ def cpu_baseline(W, U, psi, seconds=8.0):
 
     small = cg(SMALL_DIMS, W.gen_gauge(SMALL_DIMS, "haar", 0, 1), SMALL_KAPPA,
                W.gen_spinor(SMALL_DIMS, 2, W.EVEN))
+    d16 = (16, 16, 16, 16)
+    small16 = cg(d16, W.gen_gauge(d16, "smooth", 0.3, 3), 0.125, W.gen_spinor(d16, 4, W.EVEN))
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
