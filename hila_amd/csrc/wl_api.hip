@@ -526,6 +526,8 @@ extern "C" int wl_create(const int dims[4], int antiperiodic_t, int device, wl_c
                hipMemsetAsync(c->flags_d, 0, 64, c->stream) != hipSuccess ||
                hipStreamSynchronize(c->stream) != hipSuccess) {
         rc = fail(WL_E_DEVICE, "context initialisation failed");
+    } else {
+        rc = noalloc_setup(device);
     }
     if (rc) {
         wl_destroy(c);

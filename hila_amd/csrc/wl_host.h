@@ -91,6 +91,9 @@ int cg_f64_run(wl_ctx *c, const CgWork &w, void *x, size_t x_bytes, const void *
 bool cg_persistent_wanted(const wl_ctx *c);
 int cg_f64_persistent(wl_ctx *c, const CgWork &w, void *x, size_t x_bytes, const void *b, double kappa,
                       double accuracy, int maxiters, wl_cg_stats *stats, bool *ran);
+// Device address of the fixed workspace of wl_cg_solve_noalloc on `device` (wl_noalloc.hip),
+// looked up once per device with that device current; wl_create calls it.
+int noalloc_setup(int device);
 // words 32-bit words at p = 0, by a kernel (no memset node if a caller captures the stream)
 hipError_t launch_zero_words(void *p, int words, hipStream_t s);
 int ensure_partials(wl_ctx *c, size_t n);

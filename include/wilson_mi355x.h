@@ -184,6 +184,20 @@ int wl_hasenbusch_apply(wl_ctx *ctx, wl_field *out, const wl_field *in, double k
 int wl_cg_solve(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa, double accuracy,
                 int maxiters, wl_cg_stats *stats);
 
+/* wl_cg_solve for one right-hand side that allocates nothing: no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (tight solve loops,
+ * harnesses that forbid allocation during a run).  The same FP64 CG and kernels as
+ * wl_cg_solve, with the same result bit for bit, but its work memory (Krylov fields, the odd
+ * temporary, partial sums, CG state) is one fixed device array that the library holds on
+ * every device it runs on (30 MiB, until the process ends).  Limit: lattices whose parity
+ * block has at most 2^15 sites after rounding up to 256 (e.g. 16^4); larger ones return
+ * WL_E_ARG.  Needs the gauge field uploaded and, with WL_OPT_LINK_ROWS = 2, the FP64 two-row
+ * copy already made by an earlier operator or solver call (WL_E_STATE otherwise).  Calls on
+ * one device share the array: within a process they are serialised by a lock; calls from
+ * several processes each use their own copy. */
+int wl_cg_solve_noalloc(wl_ctx *ctx, wl_field *x, const wl_field *b, double kappa, double accuracy,
+                        int maxiters, wl_cg_stats *stats);
+
 /* Mixed-precision CG on M^dag M with reliable updates (an addition: HILA's only mixed
  * precision is a full FP32 CG used as the initial guess of an FP64 CG,
  * libraries/hmc/fermion_field.h:65-86).  The operator and the Krylov vectors r, p, Dp are in
