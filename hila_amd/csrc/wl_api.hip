@@ -506,6 +506,7 @@ extern "C" int wl_create(const int dims[4], int antiperiodic_t, int device, wl_c
     c->g.stride = plane_stride(c->g.vh, -1);
     c->g.antiperiodic = c->antiperiodic;
     c->g.blk = dslash_block_auto(c->g.vh);
+    c->g.dir = dir_auto(c->g.vh);
     (void)set_chunk(c->g, -1);
     DeviceGuard dg(device);
     int rc = WL_OK;
@@ -827,6 +828,12 @@ extern "C" int wl_set_option(wl_ctx *c, int option, int value) {
         if (value != 0 && value != 1) return fail(WL_E_ARG, "graphs option must be 0 or 1");
         c->use_graphs = value == 1;
         return WL_OK;
+    case WL_OPT_SMALL_HOP:
+        if (value < -1 || value > 1)
+            return fail(WL_E_ARG, "small-lattice hop option must be -1 (never), 0 (automatic) or 1 (always)");
+        c->small_hop = value;
+        c->g.dir = value == 0 ? dir_auto(c->g.vh) : (value > 0 ? 1 : 0);
+        return WL_OK;
     case WL_OPT_CG_PERSISTENT:
         if (value < -1 || value > 1)
             return fail(WL_E_ARG, "persistent CG option must be -1 (never), 0 (automatic) or 1 (always)");
@@ -877,6 +884,7 @@ extern "C" int wl_get_option(const wl_ctx *c, int option, int *value) {
     case WL_OPT_PIPELINE: *value = c->pipeline; return WL_OK;
     case WL_OPT_LINK_ROWS: *value = c->link_rows; return WL_OK;
     case WL_OPT_CG_PERSISTENT: *value = c->cg_persistent; return WL_OK;
+    case WL_OPT_SMALL_HOP: *value = c->small_hop; return WL_OK;
     default: return fail(WL_E_ARG, "unknown option %d", option);
     }
 }

@@ -556,7 +556,7 @@ int cg_batched(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec, wl_
 }  // namespace
 
 size_t cg_partials_needed(const wl_ctx *c) {
-    return (size_t)std::max((c->g.vh + c->g.blk - 1) / c->g.blk, nblocks(c->g));
+    return (size_t)std::max({(c->g.vh + c->g.blk - 1) / c->g.blk, hop_blocks(c->g), nblocks(c->g)});
 }
 
 int cg_work_of(wl_ctx *c, CgWork *w) {

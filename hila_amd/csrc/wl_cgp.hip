@@ -1,7 +1,7 @@
 // Persistent conjugate gradient on M^dag M (libraries/dirac/conjugate_gradient.h:46-104): the
 // whole FP64 single-rhs solve -- source norm, initial residual, every iteration and the final
-// true residual -- in one cooperative launch, for lattices small enough that the kernel-by-
-// kernel solve (wl_cg.hip, cg_f64) is bound by launch latency rather than memory.
+// true residual -- in one cooperative launch, with no host round trip and no kernel boundary
+// inside the solve (WL_OPT_CG_PERSISTENT = 1; the kernel-by-kernel solve is wl_cg.hip, cg_f64).
 //
 // The workgroups meet at grid barriers between the passes that read other workgroups'
 // output (a hopping pass reads neighbours; a reduction needs every partial sum).  Each pass
@@ -258,12 +258,11 @@ int persistent_grid(const wl_ctx *c) {
 
 }  // namespace
 
-// Lattices whose solve the automatic setting runs in one launch: up to 2^15 sites per parity.
-constexpr int kPersistentAutoVh = 1 << 15;
-
-bool cg_persistent_wanted(const wl_ctx *c) {
-    return c->cg_persistent > 0 || (c->cg_persistent == 0 && c->g.vh <= kPersistentAutoVh);
-}
+// Only on request: measured on MI355X the one-launch solve is not faster at any size (8^4:
+// 1.05 against 0.96 ms per solve; 16^4: 18.0 against 4.1 ms).  A grid barrier must write back
+// and invalidate each XCD's L2 (agent-scope release and acquire), which costs more than the
+// kernel boundary it replaces.
+bool cg_persistent_wanted(const wl_ctx *c) { return c->cg_persistent > 0; }
 
 __global__ void k_zero_words(unsigned *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -260,6 +260,8 @@ struct Geom {
     // ns = segments per parity (both valid only when qc > 0).
     int qc, sps, ns;
     int blk;         // workgroup size of the hopping kernel (64, 128 or 256)
+    int dir;         // 1: single-rhs hops use the small-lattice kernel (one wave per direction,
+                     // 64 sites per workgroup of 256 lanes), 0: one lane per site
     int lds;         // LDS bytes reserved per hopping workgroup (0: none); caps the workgroups
                      // sharing a CU, i.e. the sites in flight and their cache footprint
 };

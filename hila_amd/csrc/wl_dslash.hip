@@ -33,6 +33,112 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
     if constexpr (EPI == EPI_AXPY_NORM || EPI == EPI_CGUPD) reduce_partial(nrm, A.red);
 }
 
+// ------------------------------------------------------------ small lattices ------------
+// A launch of the kernel above has one lane per site, so on a small lattice it has few waves
+// (8^4: 32 waves for 1024 SIMDs), and each lane waits for about four dependent memory round
+// trips of its pipelined 8-hop chain: the launch is bound by memory latency.  Here a workgroup
+// of 4 waves covers 64 sites and wave mu computes both hops of direction mu, with the loads of
+// both issued before either is used: one round trip.  The four partial sums meet in LDS and
+// are added in a fixed order; then wave s applies the epilogue to spin component s of the 64
+// sites, with its operands loaded at the start of the kernel.
+
+template <typename R, int SIGN, int MU, int LF>
+__device__ __forceinline__ void dir_pair(cx<R> (&acc)[4][3], const DslashArgs<R> &A, int k, const SiteNb &n) {
+    const int S = A.g.stride;
+    const cx<R> *gout = A.gauge + ((size_t)A.out_par * 4 + MU) * LF * S;
+    const cx<R> *gin = A.gauge + ((size_t)(1 - A.out_par) * 4 + MU) * LF * S;
+    HopBuf<R> f, b;
+    hop_load<true, LF>(f, A.in, gout, S, n.fwd[MU], S, k);
+    hop_load<true, LF>(b, A.in, gin, S, n.bwd[MU], S, n.bwd[MU]);
+    const R sf = (MU == 3 && A.g.antiperiodic && n.fwd_edge_t) ? R(-1) : R(1);
+    const R sb = (MU == 3 && A.g.antiperiodic && n.bwd_edge_t) ? R(-1) : R(1);
+    hop_compute<MU, SIGN, true>(acc, f, sf);
+    hop_compute<MU, SIGN, false>(acc, b, sb);
+}
+
+template <typename R, int SIGN, int EPI, int LF>
+__global__ __launch_bounds__(256) void k_dslash_dir(DslashArgs<R> A) {
+    if (A.done && *A.done) return;
+    __shared__ cx<R> part[4][12][64];  // [direction][component][site]
+    const int w = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
+    const int k = xcd_tile(blockIdx.x, gridDim.x) * 64 + l;
+    const bool live = k < A.g.vh;
+    const int S = A.g.stride;
+    size_t e[3], ex[3];  // elements of spin component w of site k (storage, FP64 solution)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        e[c] = comp_elem<R>(3 * w + c, S, k);
+        ex[c] = comp_elem<double>(3 * w + c, S, k);
+    }
+    cx<R> xi[3], pv[3], rv[3];
+    cx<double> xv[3];
+    if (live) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if constexpr (EPI != EPI_SET) xi[c] = ldc<true>(A.xin + e[c]);
+            if constexpr (EPI == EPI_CGUPD) {
+                pv[c] = ldc<true>(A.p + e[c]);
+                rv[c] = ldc<true>(A.r + e[c]);
+                xv[c] = ldc<true>(A.x + ex[c]);
+            }
+        }
+        const SiteNb n = neighbours(A.g, k, A.out_par);
+        cx<R> acc[4][3];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[s][c] = {R(0), R(0)};
+        switch (w) {  // wave-uniform
+        case 0: dir_pair<R, SIGN, 0, LF>(acc, A, k, n); break;
+        case 1: dir_pair<R, SIGN, 1, LF>(acc, A, k, n); break;
+        case 2: dir_pair<R, SIGN, 2, LF>(acc, A, k, n); break;
+        default: dir_pair<R, SIGN, 3, LF>(acc, A, k, n); break;
+        }
+#pragma unroll
+        for (int c = 0; c < 12; ++c) part[w][c][l] = acc[c / 3][c % 3];
+    }
+    __syncthreads();
+    double nrm = 0;
+    if (live) {
+        const R a = A.a;
+        cx<R> h[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int q = 3 * w + c;
+            h[c] = (part[0][q][l] + part[1][q][l]) + (part[2][q][l] + part[3][q][l]);
+        }
+        if constexpr (EPI == EPI_SET) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) stc<true>(A.out + e[c], a * h[c]);
+        } else if constexpr (EPI == EPI_AXPY || EPI == EPI_AXPY_NORM) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                cx<R> v = sizeof(R) == 8 ? cx<R>{(R)__builtin_fma(A.b, xi[c].x, 0.0), (R)__builtin_fma(A.b, xi[c].y, 0.0)}
+                                         : cx<R>{__builtin_fmaf(A.b, xi[c].x, 0.0f), __builtin_fmaf(A.b, xi[c].y, 0.0f)};
+                v = v + a * h[c];
+                if constexpr (EPI == EPI_AXPY_NORM) nrm += (double)norm2(v);
+                stc<true>(A.out + e[c], v);
+            }
+        } else {  // EPI_CGUPD: x (FP64) += alpha p, r -= alpha DDp
+            const double alpha = A.red.st->alpha;
+            const R alpha_r = (R)alpha;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const cx<R> ddp = sizeof(R) == 8
+                                      ? cx<R>{(R)__builtin_fma(A.b, xi[c].x, 0.0), (R)__builtin_fma(A.b, xi[c].y, 0.0)}
+                                      : cx<R>{__builtin_fmaf(A.b, xi[c].x, 0.0f), __builtin_fmaf(A.b, xi[c].y, 0.0f)};
+                const cx<double> pd = {(double)pv[c].x, (double)pv[c].y};
+                stc<true>(A.x + ex[c], xv[c] + alpha * pd);
+                const cx<R> dd = ddp + a * h[c];
+                const cx<R> r = rv[c] - alpha_r * dd;
+                nrm += (double)norm2(r);
+                stc<true>(A.r + e[c], r);
+            }
+        }
+    }
+    if constexpr (EPI == EPI_AXPY_NORM || EPI == EPI_CGUPD) reduce_partial(nrm, A.red);
+}
+
 // Default workgroup size, from measurements on MI355X (256 CUs): 256 lanes from 2^19 sites
 // per parity up, 64 lanes for mid-size lattices, 128 below (where a launch is latency-bound).
 int dslash_block_auto(int vh) {
@@ -42,6 +148,10 @@ int dslash_block_auto(int vh) {
 }
 
 int nblocks(const Geom &g) { return (g.vh + kBlock - 1) / kBlock; }
+
+int hop_blocks(const Geom &g) { return g.dir ? (g.vh + 63) / 64 : (g.vh + g.blk - 1) / g.blk; }
+
+int dir_auto(int vh) { return vh <= kDirAutoVh ? 1 : 0; }
 
 template <typename R, int SIGN, int EPI>
 static hipError_t launch_t(const Geom &g, const DslashLaunch &L, hipStream_t s) {
@@ -60,12 +170,17 @@ static hipError_t launch_t(const Geom &g, const DslashLaunch &L, hipStream_t s) 
     A.r = (cx<R> *)L.r;
     A.red = L.red;
     A.fin = L.fin;
-    const int blk = g.blk;
-    const int nb = (g.vh + blk - 1) / blk;
-    if (L.lf == 6)
-        hipLaunchKernelGGL((k_dslash<R, SIGN, EPI, 6>), dim3(nb), dim3(blk), g.lds, s, A);
-    else
-        hipLaunchKernelGGL((k_dslash<R, SIGN, EPI, 9>), dim3(nb), dim3(blk), g.lds, s, A);
+    const int nb = hop_blocks(g);
+    if (g.dir) {
+        if (L.lf == 6)
+            hipLaunchKernelGGL((k_dslash_dir<R, SIGN, EPI, 6>), dim3(nb), dim3(256), 0, s, A);
+        else
+            hipLaunchKernelGGL((k_dslash_dir<R, SIGN, EPI, 9>), dim3(nb), dim3(256), 0, s, A);
+    } else if (L.lf == 6) {
+        hipLaunchKernelGGL((k_dslash<R, SIGN, EPI, 6>), dim3(nb), dim3(g.blk), g.lds, s, A);
+    } else {
+        hipLaunchKernelGGL((k_dslash<R, SIGN, EPI, 9>), dim3(nb), dim3(g.blk), g.lds, s, A);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || (EPI != EPI_AXPY_NORM && EPI != EPI_CGUPD)) return e;
     return launch_finish(L.red, nb, L.fin, L.done, s);

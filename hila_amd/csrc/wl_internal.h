@@ -155,6 +155,11 @@ hipError_t launch_finish(const RedArgs &ra, int nparts, int fin, const int *done
 
 int nblocks(const Geom &g);
 int dslash_block_auto(int vh);
+// workgroups of a single-rhs hopping launch (one partial sum each in the norm epilogues)
+int hop_blocks(const Geom &g);
+// Geom::dir by default: the small-lattice hopping kernel up to kDirAutoVh sites per parity
+constexpr int kDirAutoVh = 1 << 15;
+int dir_auto(int vh);
 
 }  // namespace wl
 
@@ -223,9 +228,11 @@ struct wl_ctx {
     int link_load = 0;                 // multi-rhs links: 0 automatic, 1 cached, 2 nontemporal, 3 LDS
     bool use_graphs = false;           // batched CG: replay each chunk from a captured HIP graph
     int pipeline = 0;                  // multi-rhs hops: 0 pipelined where possible, 1 never
-    int cg_persistent = 0;             // single-rhs FP64 CG in one launch: -1 never, 0 small
-                                       // lattices, 1 always (where the launch fits)
+    int cg_persistent = 0;             // single-rhs FP64 CG in one launch: 1 yes (where the
+                                       // launch fits), 0 (default) or -1 kernel by kernel
     void *grid_bar = nullptr;          // grid barrier of the persistent CG (on first use)
+    int small_hop = 0;                 // WL_OPT_SMALL_HOP (-1 never, 0 automatic, 1 always);
+                                       // the setting in effect is g.dir
     bool blk_auto = true;              // g.blk is the volume's default (not chosen by a caller)
     bool lds_auto = true;              // g.lds unset: multi-rhs hops take mr_geom's occupancy cap
     double op_shift = 0.0;             // Hasenbusch shift mh of the operator being applied,

@@ -337,11 +337,21 @@ int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, d
  *       1 = the whole solve -- source norm, initial residual, every iteration and the final
  *       true residual -- runs as one cooperative launch whose workgroups meet at grid
  *       barriers, with no host round trip inside the solve; -1 = kernel by kernel, with the
- *       host checking convergence every 8 iterations; 0 (default) = one launch on lattices of
- *       at most 2^15 sites per parity, where a solve is bound by launch latency.  Both give
- *       the same solution, bit for bit: the persistent launch runs the same per-site code and
- *       sums every reduction in the same order.  Falls back to kernel by kernel when the
- *       launch does not fit on the device at once. */
+ *       host checking convergence every 8 iterations; 0 (default) = kernel by kernel: on
+ *       MI355X the grid barriers of the one-launch solve (an L2 write-back and invalidate per
+ *       XCD each) cost more than the kernel boundaries they replace, at every lattice size.
+ *       Both give the same solution, bit for bit: the persistent launch runs the same per-site
+ *       code and sums every reduction in the same order.  Falls back to kernel by kernel when
+ *       the launch does not fit on the device at once.  The one-launch solve always runs one
+ *       lane per site: it equals the kernel-by-kernel solve bit for bit with
+ *       WL_OPT_SMALL_HOP = -1.
+ *   WL_OPT_SMALL_HOP  single-rhs hopping launches (wl_dslash_eo, wl_meo_apply and the
+ *       single-rhs solvers): 1 = the small-lattice kernel, in which each of 4 waves computes
+ *       the two hops of one direction for 64 sites (loads of both issued together: one memory
+ *       round trip) and the partial sums are added in LDS; -1 = one lane per site, the 8 hops
+ *       pipelined; 0 (default) = the small-lattice kernel up to 2^15 sites per parity, where a
+ *       one-lane launch has too few waves to hide memory latency.  Results agree to rounding
+ *       (the hops are summed in another order). */
 enum {
     WL_OPT_TRAVERSAL_CHUNK = 1,
     WL_OPT_BLOCK = 2,
@@ -353,7 +363,8 @@ enum {
     WL_OPT_GRAPHS = 8,
     WL_OPT_PIPELINE = 9,
     WL_OPT_LINK_ROWS = 10,
-    WL_OPT_CG_PERSISTENT = 11
+    WL_OPT_CG_PERSISTENT = 11,
+    WL_OPT_SMALL_HOP = 12
 };
 int wl_set_option(wl_ctx *ctx, int option, int value);
 int wl_get_option(const wl_ctx *ctx, int option, int *value);

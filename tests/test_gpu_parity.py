@@ -228,6 +228,7 @@ def test_traversal_and_workgroup_options_do_not_change_results(dims):
     size change only the speed: the output is bitwise identical."""
     lat, U, _ = _setup(dims, "smooth", 0.3, 3, True)
     del U
+    lat.set_option(W.OPT_SMALL_HOP, -1)  # the one-lane kernel, whose traversal these options set
     D = W.Dirac_Wilson_evenodd(0.125, lat)
     psi = W.gen_spinor(dims, 4, EVEN)
     fin, fo, fe = W.Field(lat).upload(psi), W.Field(lat, ODD), W.Field(lat)

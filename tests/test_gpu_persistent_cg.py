@@ -20,11 +20,13 @@ CASES = [
     ((16, 16, 16, 16), "smooth", 0.3, 3, 0.125, True, 4),
     ((6, 6, 6, 10), "smooth", 0.3, 61, 0.12, True, 5),
     ((4, 6, 8, 10), "smooth", 0.5, 9, 0.11, False, 6),
-    ((20, 20, 20, 20), "smooth", 0.3, 7, 0.124, True, 8),  # beyond the automatic size
+    ((20, 20, 20, 20), "smooth", 0.3, 7, 0.124, True, 8),
 ]
 
 
 def solve(lat, op, b, mode, tol=1e-10, maxiters=10000):
+    # the one-launch solve runs one lane per site: compare it with the same kernel by kernel
+    lat.set_option(W.OPT_SMALL_HOP, -1)
     lat.set_option(W.OPT_CG_PERSISTENT, mode)
     x = W.Field(lat)
     t = time.perf_counter()
