@@ -24,7 +24,7 @@ LIB = os.path.join(LIBDIR, "libhila_wilson.so")
 
 HIP_SOURCES = ["wl_rb_f16.hip", "wl_rb_f32.hip", "wl_rb_f32r.hip", "wl_rb_f64.hip", "wl_rb_f64r.hip",
                "wl_dslash.hip", "wl_multi.hip",
-               "wl_blas.hip", "wl_api.hip", "wl_cg.hip", "wl_cgp.hip", "wl_noalloc.hip", "wl_serve.hip", "wl_config.hip", "wl_mre.hip", "wl_force.hip"]
+               "wl_blas.hip", "wl_api.hip", "wl_cg.hip", "wl_cgp.hip", "wl_bcg.hip", "wl_noalloc.hip", "wl_serve.hip", "wl_config.hip", "wl_mre.hip", "wl_force.hip"]
 # Per-source device flags.  wl_rb_f16.hip: without the v_dot2c_f32_f16 encoding (dot5-insts,
 # which accumulates in place and so needs a zero move to start every FP16 dot-product chain)
 # the compiler selects v_dot2_f32_f16, whose accumulator is a separate operand (an inline 0 at

@@ -828,6 +828,10 @@ extern "C" int wl_set_option(wl_ctx *c, int option, int value) {
         if (value != 0 && value != 1) return fail(WL_E_ARG, "graphs option must be 0 or 1");
         c->use_graphs = value == 1;
         return WL_OK;
+    case WL_OPT_BLOCK_KRYLOV:
+        if (value != 0 && value != 1) return fail(WL_E_ARG, "block Krylov option must be 0 or 1");
+        c->block_krylov = value == 1;
+        return WL_OK;
     case WL_OPT_SMALL_HOP:
         if (value < -1 || value > 1)
             return fail(WL_E_ARG, "small-lattice hop option must be -1 (never), 0 (automatic) or 1 (always)");
@@ -885,6 +889,7 @@ extern "C" int wl_get_option(const wl_ctx *c, int option, int *value) {
     case WL_OPT_LINK_ROWS: *value = c->link_rows; return WL_OK;
     case WL_OPT_CG_PERSISTENT: *value = c->cg_persistent; return WL_OK;
     case WL_OPT_SMALL_HOP: *value = c->small_hop; return WL_OK;
+    case WL_OPT_BLOCK_KRYLOV: *value = c->block_krylov ? 1 : 0; return WL_OK;
     default: return fail(WL_E_ARG, "unknown option %d", option);
     }
 }

@@ -637,6 +637,7 @@ int cg_solve(wl_ctx *c, wl_field *x, const wl_field *b, const wl_cg_params &p, b
     s.mixed = mixed;
     s.delta = p.delta;
     s.update_every = p.update_every;
+    if (c->block_krylov && b->nrhs > 1) return cg_block(c, x->data, b->data, b->nrhs, s, stats);
     return cg_batched(c, x->data, b->data, b->nrhs, s, stats);
 }
 

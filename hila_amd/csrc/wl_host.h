@@ -113,7 +113,7 @@ int meo_one(wl_ctx *c, const RedArgs &ra, int prec, void *out, const void *in, v
 
 // Work fields of the solvers, one per (slot, precision, parity), created on first use with at
 // least the requested number of right-hand sides and kept resident with the context.
-enum WorkSlot { W_R = 0, W_P, W_DP, W_T, W_DP64, W_DDP64, W_T64, W_X, W_Y, W_SRC };
+enum WorkSlot { W_R = 0, W_P, W_DP, W_T, W_DP64, W_DDP64, W_T64, W_X, W_Y, W_SRC, W_BZ };
 int work_field(wl_ctx *c, int slot, int prec, int parity, int nrhs, wl_field **out);
 
 int rhs_per_lane(const wl_ctx *c, int prec, int N);

@@ -231,6 +231,7 @@ struct wl_ctx {
     int cg_persistent = 0;             // single-rhs FP64 CG in one launch: 1 yes (where the
                                        // launch fits), 0 (default) or -1 kernel by kernel
     void *grid_bar = nullptr;          // grid barrier of the persistent CG (on first use)
+    bool block_krylov = false;         // WL_OPT_BLOCK_KRYLOV: several rhs share one Krylov space
     int small_hop = 0;                 // WL_OPT_SMALL_HOP (-1 never, 0 automatic, 1 always);
                                        // the setting in effect is g.dir
     bool blk_auto = true;              // g.blk is the volume's default (not chosen by a caller)

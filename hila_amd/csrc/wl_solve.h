@@ -38,6 +38,10 @@ struct CgSpec {
     int update_every = 100;
 };
 
+// Block CG (wl_bcg.hip): N <= 24 right-hand sides sharing one Krylov space, FP64 or FP32 inner
+// precision (reliable updates when spec.mixed); stats: N entries.
+int cg_block(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec, wl_cg_stats *stats);
+
 class BatchCG {
   public:
     BatchCG() = default;

@@ -30,6 +30,7 @@ EVEN, ODD = 1, 2
 F64, F32, F16 = 0, 1, 2
 OPT_TRAVERSAL_CHUNK, OPT_BLOCK, OPT_RHS_BLOCK, OPT_RHS_LAUNCH, OPT_PLANE_PAD, OPT_LINK_LOAD = 1, 2, 3, 4, 5, 6
 OPT_LDS_RESERVE, OPT_GRAPHS, OPT_PIPELINE, OPT_LINK_ROWS, OPT_CG_PERSISTENT, OPT_SMALL_HOP = 7, 8, 9, 10, 11, 12
+OPT_BLOCK_KRYLOV = 13
 
 EXPORTED_SYMBOLS = [
     "wl_version", "wl_last_error", "wl_create", "wl_destroy", "wl_sync", "wl_volume",

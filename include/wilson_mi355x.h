@@ -351,7 +351,14 @@ int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, d
  *       round trip) and the partial sums are added in LDS; -1 = one lane per site, the 8 hops
  *       pipelined; 0 (default) = the small-lattice kernel up to 2^15 sites per parity, where a
  *       one-lane launch has too few waves to hide memory latency.  Results agree to rounding
- *       (the hops are summed in another order). */
+ *       (the hops are summed in another order).
+ *   WL_OPT_BLOCK_KRYLOV  CG on a field of 2..24 right-hand sides (wl_cg_solve, wl_cg_solve_mixed
+ *       and wl_cg_solve_ex with an FP64 or FP32 inner precision): 1 = block CG, the rhs share
+ *       one Krylov space (the residual block is kept orthonormal, R = Q C; dependent or zero
+ *       directions are dropped from the block), every rhs ends at its FP64 true residual, and
+ *       every stats entry reports the block's iteration count; 0 (default) = an independent
+ *       CG recurrence per rhs sharing the operator applications (batched CG).  The request
+ *       server always solves batched. */
 enum {
     WL_OPT_TRAVERSAL_CHUNK = 1,
     WL_OPT_BLOCK = 2,
@@ -364,7 +371,8 @@ enum {
     WL_OPT_PIPELINE = 9,
     WL_OPT_LINK_ROWS = 10,
     WL_OPT_CG_PERSISTENT = 11,
-    WL_OPT_SMALL_HOP = 12
+    WL_OPT_SMALL_HOP = 12,
+    WL_OPT_BLOCK_KRYLOV = 13
 };
 int wl_set_option(wl_ctx *ctx, int option, int value);
 int wl_get_option(const wl_ctx *ctx, int option, int *value);

@@ -256,6 +256,20 @@ def test_c4_batched_24_rhs():
         s64 = W.CG(D, tol, 10000).apply(b1, x1)[0]
         assert s64["converged"] == 1
         assert rel(x.download(j), x1.download()) <= 1e-8
+    # the same 24 rhs by block CG (one shared Krylov space), FP32 inner, FP64 reliable updates
+    say("C4 block solve")
+    lat.set_option(W.OPT_BLOCK_KRYLOV, 1)
+    xb = W.Field(lat, EVEN, W.F64, nrhs=N)
+    sb = W.CG(D, tol, 10000, precision="f32", delta=1e-3, update_every=100).apply(b, xb)
+    lat.set_option(W.OPT_BLOCK_KRYLOV, 0)
+    say("C4 block", sb[0]["iterations"], "iterations, batched", max(q["iterations"] for q in st))
+    for j, q in enumerate(sb):
+        assert q["converged"] == 1 and q["true_resid"] <= tol, (j, q)
+        assert rel(xb.download(j), x.download(j)) <= 1e-8, j
+    assert sb[0]["iterations"] < max(q["iterations"] for q in st)
+    xj = xb.to_full(12)
+    r = srcs[12] - orc.meo(U, kappa, orc.meo(U, kappa, xj, 1), -1)
+    assert float(np.linalg.norm(r) / np.linalg.norm(srcs[12])) <= 1.05 * tol
     lat.close()
 
 
