@@ -43,22 +43,45 @@ constexpr int kBcgMax = 24;    // right-hand sides of one block
 // laid out as the FP64 solution is, one complex number per component plane: unpacked FP32.
 struct f32u {};
 constexpr int kF32Unpacked = 100;  // its precision code in launch_bcombo
-constexpr int kGramWg = 512;   // workgroups of a Gram reduction (partials)
+constexpr int kGramWg = 1024;  // workgroups of a Gram reduction (partials)
+
+// The 16-byte vector q (components q P .. q P + P - 1, P = St<T>::pack) of site k of rhs block
+// `base`, widened to the arithmetic type; VP = 16-byte vectors per spinor (12 / P).
+template <typename T>
+struct Vec16 {
+    static constexpr int P = St<T>::pack, VP = 12 / P;
+    cx<typename St<T>::R> e[P];
+};
+template <typename T>
+__device__ __forceinline__ Vec16<T> ld_vec16(const typename St<T>::E *base, int q, int S, int k) {
+    const auto raw = ld_vec<false, T>(base + (size_t)q * St<T>::pack * S + (size_t)St<T>::pack * k);
+    Vec16<T> v;
+#pragma unroll
+    for (int h = 0; h < St<T>::pack; ++h) v.e[h] = widen(raw.e[h]);
+    return v;
+}
 
 // G[i][j] = sum over sites and components of conj(U_i) V_j (FP64 accumulation), i, j < N, for
-// N-rhs fields U, V in storage T.  A workgroup takes tiles of 64 sites; per component it stages
-// the tile of every rhs of U and V in LDS, and thread (bi, bj, g) accumulates the 3 x 3 block
-// (3bi.., 3bj..) of G over the 16 sites of group g.  One partial G per workgroup.
+// N-rhs fields U, V in storage T (FP64 or FP32).  A workgroup takes tiles of 64 sites; per
+// 16-byte vector q of the spinor it stages the tile of every rhs of U and V in LDS (the next
+// vector's loads are issued before the current one is used), and thread (bi, bj, g)
+// accumulates the 3 x 3 block (3bi.., 3bj..) of G over the 16 sites of group g.  One partial
+// G per workgroup.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bgram(const typename St<T>::E *__restrict__ U,
                                                const typename St<T>::E *__restrict__ V, int N, int vh, int S,
                                                cx<double> *__restrict__ partials) {
     using R = typename St<T>::R;
+    using Vt = Vec16<T>;
+    constexpr int P = Vt::P, VP = Vt::VP;
+    constexpr int PER = kBcgMax * 64 / 256;  // vectors per field a thread stages
     // the staged tiles, then (after the last tile) the four site groups' sums
-    constexpr size_t kTiles = (size_t)2 * kBcgMax * 64 * sizeof(cx<R>), kRed = (size_t)4 * 64 * 9 * sizeof(cx<double>);
+    // (rows padded to 65 vectors: the 8 rows a wave reads at once then fall in different banks)
+    constexpr int RW = 65;
+    constexpr size_t kTiles = (size_t)2 * kBcgMax * RW * sizeof(Vt), kRed = (size_t)4 * 64 * 9 * sizeof(cx<double>);
     __shared__ __align__(16) unsigned char smem[kTiles > kRed ? kTiles : kRed];
-    auto su = reinterpret_cast<cx<R>(*)[64]>(smem);
-    auto sv = reinterpret_cast<cx<R>(*)[64]>(smem + (size_t)kBcgMax * 64 * sizeof(cx<R>));
+    auto su = reinterpret_cast<Vt(*)[RW]>(smem);
+    auto sv = reinterpret_cast<Vt(*)[RW]>(smem + (size_t)kBcgMax * RW * sizeof(Vt));
     auto red = reinterpret_cast<cx<double>(*)[64][9]>(smem);
     const int t = threadIdx.x, blk = t & 63, g = t >> 6;
     const int bi = blk >> 3, bj = blk & 7;
@@ -69,27 +92,42 @@ __global__ __launch_bounds__(256) void k_bgram(const typename St<T>::E *__restri
 #pragma unroll
         for (int b = 0; b < 3; ++b) acc[a][b] = {0.0, 0.0};
     const int ntiles = (vh + 63) / 64;
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        for (int c = 0; c < 12; ++c) {
-            for (int e = t; e < kBcgMax * 64; e += 256) {
-                const int i = e >> 6, l = e & 63, k = tile * 64 + l;
-                cx<R> u = {R(0), R(0)}, v = {R(0), R(0)};
-                if (i < N && k < vh) {
-                    const size_t el = (size_t)i * rs + comp_elem<T>(c, S, k);
-                    u = U[el];
-                    v = V[el];
-                }
-                su[i][l] = u;
-                sv[i][l] = v;
+    const int nsteps = ((ntiles - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x) * VP;
+    // step = (tile, vector q); the loads of step + 1 are in flight while step is computed
+    Vt pu[PER], pv[PER];
+    auto load = [&](int step) {
+        const int tile = (int)blockIdx.x + (step / VP) * (int)gridDim.x, q = step % VP;
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int e = t + 256 * r, i = e >> 6, k = tile * 64 + (e & 63);
+            if (i < N && k < vh) {
+                pu[r] = ld_vec16<T>(U + (size_t)i * rs, q, S, k);
+                pv[r] = ld_vec16<T>(V + (size_t)i * rs, q, S, k);
+            } else {
+#pragma unroll
+                for (int h = 0; h < P; ++h) pu[r].e[h] = pv[r].e[h] = {R(0), R(0)};
             }
-            __syncthreads();
-#pragma unroll 4
-            for (int s = 0; s < 16; ++s) {
-                const int l = g * 16 + s;
+        }
+    };
+    if (nsteps > 0) load(0);
+    for (int step = 0; step < nsteps; ++step) {
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int e = t + 256 * r;
+            su[e >> 6][e & 63] = pu[r];
+            sv[e >> 6][e & 63] = pv[r];
+        }
+        __syncthreads();
+        if (step + 1 < nsteps) load(step + 1);
+#pragma unroll 2
+        for (int s2 = 0; s2 < 16; ++s2) {
+            const int l = g * 16 + s2;
+#pragma unroll
+            for (int h = 0; h < P; ++h) {
                 cx<double> u[3], v[3];
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    const cx<R> uu = su[3 * bi + a][l], vv = sv[3 * bj + a][l];
+                    const cx<R> uu = su[3 * bi + a][l].e[h], vv = sv[3 * bj + a][l].e[h];
                     u[a] = {(double)uu.x, (double)uu.y};
                     v[a] = {(double)vv.x, (double)vv.y};
                 }
@@ -102,11 +140,10 @@ __global__ __launch_bounds__(256) void k_bgram(const typename St<T>::E *__restri
                         acc[a][b].y += u[a].x * v[b].y - u[a].y * v[b].x;
                     }
             }
-            __syncthreads();
         }
+        __syncthreads();
     }
-    // sum the four site groups (fixed order) through LDS (the last tile's reads are done: the
-    // loop ends with a barrier)
+    // sum the four site groups (fixed order) through LDS (the loop ends with a barrier)
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -125,13 +162,15 @@ __global__ __launch_bounds__(256) void k_bgram(const typename St<T>::E *__restri
     }
 }
 
-// out[i][j] = sum over the nwg partial G, in workgroup order
-__global__ void k_bgram_finish(const cx<double> *__restrict__ partials, int nwg, int N, cx<double> *out) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= N * N) return;
+// out[e] = sum over the nwg partial G of entry e: one workgroup per entry, a fixed-order
+// reduction over the workgroup's lanes
+__global__ __launch_bounds__(kBlock) void k_bgram_finish(const cx<double> *__restrict__ partials, int nwg, int N,
+                                                         cx<double> *out) {
+    const int e = blockIdx.x, NN = N * N;
     cx<double> s = {0.0, 0.0};
-    for (int w = 0; w < nwg; ++w) s = s + partials[(size_t)w * N * N + e];
-    out[e] = s;
+    for (int w = threadIdx.x; w < nwg; w += blockDim.x) s = s + partials[(size_t)w * NN + e];
+    const double re = block_sum(s.x), im = block_sum(s.y);
+    if (threadIdx.x == 0) out[e] = {re, im};
 }
 
 // Out_j = sum_i In_i A[i][j] + gamma Add_j for every site and component (i, j < N), with A an
@@ -139,6 +178,41 @@ __global__ void k_bgram_finish(const cx<double> *__restrict__ partials, int nwg,
 // (2v, 2v + 1): one 16-byte vector of an FP32 field, two of an FP64 one.  Out may be In (every
 // In_i of the thread's components is loaded before any Out_j is stored) and may be Add.
 // Arithmetic in the wider of the In and Out storage types.
+// Components (c0, c0 + 1) of site k in rhs block `base`: one 16-byte load where the storage
+// packs them together (FP32), else two.
+template <typename T, typename R>
+__device__ __forceinline__ void ld_pair(const typename St<T>::E *base, int S, int k, int c0, cx<R> (&v)[2]) {
+    if constexpr (St<T>::pack == 2) {
+        const auto raw = ld_vec<false, T>(base + comp_elem<T>(c0, S, k));
+        v[0] = {(R)raw.e[0].x, (R)raw.e[0].y};
+        v[1] = {(R)raw.e[1].x, (R)raw.e[1].y};
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const auto a = base[comp_elem<T>(c0 + h, S, k)];
+            v[h] = {(R)a.x, (R)a.y};
+        }
+    }
+}
+template <typename T, typename R>
+__device__ __forceinline__ void st_pair(typename St<T>::E *base, int S, int k, int c0, const cx<R> (&v)[2]) {
+    using RO = typename St<T>::R;
+    if constexpr (St<T>::pack == 2) {
+        Packed<typename St<T>::E, 2> raw;
+        raw.e[0] = {(RO)v[0].x, (RO)v[0].y};
+        raw.e[1] = {(RO)v[1].x, (RO)v[1].y};
+        st_vec<false, T>(base + comp_elem<T>(c0, S, k), raw);
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) base[comp_elem<T>(c0 + h, S, k)] = {(RO)v[h].x, (RO)v[h].y};
+    }
+}
+
+// Out_j = sum_i In_i A[i][j] + gamma Add_j for every site and component (i, j < N), with A an
+// N x N complex FP64 matrix (row-major), held in LDS in the arithmetic type.  A thread takes
+// one site and the component pair (2v, 2v + 1): one 16-byte vector of an FP32 field, two
+// loads of an FP64 one.  Out may be In (every In_i of the thread's components is loaded before
+// any Out_j is stored) and may be Add.  Arithmetic in the wider of the In and Out types.
 template <typename TI, typename TO, typename TA>
 __global__ __launch_bounds__(256) void k_bcombo(const typename St<TI>::E *In, typename St<TO>::E *Out,
                                                 const typename St<TA>::E *Add, double gamma,
@@ -146,51 +220,37 @@ __global__ __launch_bounds__(256) void k_bcombo(const typename St<TI>::E *In, ty
     using RI = typename St<TI>::R;
     using RO = typename St<TO>::R;
     using R = typename std::conditional<(sizeof(RI) > sizeof(RO)), RI, RO>::type;
-    __shared__ cx<double> A[kBcgMax * kBcgMax];
-    for (int e = threadIdx.x; e < N * N; e += blockDim.x) A[e] = Am[e];
+    __shared__ cx<R> A[kBcgMax * kBcgMax];
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) A[e] = {(R)Am[e].x, (R)Am[e].y};
     __syncthreads();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int c0 = 2 * (int)blockIdx.y;  // components c0, c0 + 1
     if (k >= vh) return;
     const size_t rsi = (size_t)St<TI>::planes * S, rso = (size_t)St<TO>::planes * S,
                  rsa = (size_t)St<TA>::planes * S;
-    size_t ei[2], eo[2], ea[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        ei[h] = comp_elem<TI>(c0 + h, S, k);
-        eo[h] = comp_elem<TO>(c0 + h, S, k);
-        ea[h] = comp_elem<TA>(c0 + h, S, k);
-    }
     cx<R> w[kBcgMax][2];
 #pragma unroll
     for (int i = 0; i < kBcgMax; ++i)
-        if (i < N)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const cx<RI> v = In[(size_t)i * rsi + ei[h]];
-                w[i][h] = {(R)v.x, (R)v.y};
-            }
+        if (i < N) ld_pair<TI, R>(In + (size_t)i * rsi, S, k, c0, w[i]);
+    const R gm = (R)gamma;
     for (int j = 0; j < N; ++j) {
         cx<R> s[2] = {{R(0), R(0)}, {R(0), R(0)}};
-        if (Add)
+        if (Add) {
+            ld_pair<TA, R>(Add + (size_t)j * rsa, S, k, c0, s);
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const auto a = Add[(size_t)j * rsa + ea[h]];
-                s[h] = {(R)gamma * (R)a.x, (R)gamma * (R)a.y};
-            }
+            for (int h = 0; h < 2; ++h) s[h] = {gm * s[h].x, gm * s[h].y};
+        }
 #pragma unroll
         for (int i = 0; i < kBcgMax; ++i)
             if (i < N) {
-                const cx<double> m = A[i * N + j];
-                const R mx = (R)m.x, my = (R)m.y;
+                const cx<R> m = A[i * N + j];
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    s[h].x += w[i][h].x * mx - w[i][h].y * my;
-                    s[h].y += w[i][h].x * my + w[i][h].y * mx;
+                    s[h].x += w[i][h].x * m.x - w[i][h].y * m.y;
+                    s[h].y += w[i][h].x * m.y + w[i][h].y * m.x;
                 }
             }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) Out[(size_t)j * rso + eo[h]] = cx<RO>{(RO)s[h].x, (RO)s[h].y};
+        st_pair<TO, R>(Out + (size_t)j * rso, S, k, c0, s);
     }
 }
 
@@ -255,8 +315,8 @@ struct BlockCG {
             hipLaunchKernelGGL(k_bgram<float>, dim3(nwg), dim3(256), 0, S, (const cx<float> *)U, (const cx<float> *)V,
                                N, c->g.vh, c->g.stride, (cx<double> *)c->partials);
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_bgram_finish, dim3((N * N + 63) / 64), dim3(64), 0, S, (const cx<double> *)c->partials,
-                           nwg, N, dgram);
+        hipLaunchKernelGGL(k_bgram_finish, dim3(N * N), dim3(kBlock), 0, S, (const cx<double> *)c->partials, nwg, N,
+                           dgram);
         HIPCHK(hipGetLastError());
         hbuf.resize((size_t)N * N);
         HIPCHK(hipMemcpyAsync(hbuf.data(), dgram, (size_t)N * N * sizeof(cx<double>), hipMemcpyDeviceToHost, S));
