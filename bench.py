@@ -528,6 +528,47 @@ def profile_multi(W, steps):
     lat.close()
 
 
+def profile_cg_small(W, steps):
+    """`steps` FP64 CG solves to 1e-10 at 8^4 (Haar links seed 1, kappa 0.12, source seed 2),
+    kernel by kernel with the default small-lattice hopping kernel."""
+    dims = SMALL_DIMS
+    lat = W.Lattice(dims)
+    lat.upload_gauge(W.gen_gauge(dims, "haar", 0, 1))
+    D = W.Dirac_Wilson_evenodd(SMALL_KAPPA, lat)
+    b = W.Field(lat).upload(W.gen_spinor(dims, 2, W.EVEN))
+    x = W.Field(lat)
+    for _ in range(steps):
+        x.zero()
+        W.CG(D, 1e-10, 10000).apply(b, x)
+    print(f"# profile-cg-small: {steps} solves", file=sys.stderr, flush=True)
+    lat.close()
+
+
+def profile_block(W, maxiters):
+    """The 24-rhs configuration (32^4, links exp(i 0.3 H) seed 5, kappa 0.125, 12 point + 12
+    Gaussian sources) solved by block CG, FP32 inner with FP64 reliable updates, for at most
+    `maxiters` iterations (0: to 1e-10)."""
+    dims, N = (32, 32, 32, 32), 24
+    lat = W.Lattice(dims)
+    lat.upload_gauge(W.gen_gauge(dims, "smooth", 0.3, 5))
+    D = W.Dirac_Wilson_evenodd(0.125, lat)
+    b = W.Field(lat, W.EVEN, W.F64, nrhs=N)
+    pseed = even_point_seed(W, dims, 8)
+    for sc in range(12):
+        pt, _ = W.gen_point(dims, pseed, sc // 3, sc % 3)
+        b.upload(pt, rhs=sc)
+    for j in range(12):
+        b.upload(W.gen_spinor(dims, 100 + j, W.EVEN), rhs=12 + j)
+    lat.set_option(W.OPT_BLOCK_KRYLOV, 1)
+    x = W.Field(lat, W.EVEN, W.F64, nrhs=N)
+    t = time.perf_counter()
+    st = W.CG(D, 1e-10, maxiters if maxiters > 0 else 10000, precision="f32", delta=1e-3,
+              update_every=100).apply(b, x)
+    print(f"# profile-block: {st[0]['iterations']} iterations, {1e3 * (time.perf_counter() - t):.1f} ms",
+          file=sys.stderr, flush=True)
+    lat.close()
+
+
 def profile_mre(W, steps):
     """MRE guesses at 32^4 (links exp(i 0.3 H) seed 3, kappa 0.124) from 4 and from 8 Gaussian
     basis vectors, `steps` times each: the multi-rhs M^dag M on the basis, the inner-product
@@ -772,6 +813,11 @@ def main(argv=None):
                     help="short run for rocprofv3: --steps autotuned multi-rhs hopping launches "
                          "(12 rhs, FP16 then FP32 storage) at 32^4, then --steps copies of one "
                          "12-rhs FP16 field; prints no JSON line")
+    ap.add_argument("--profile-cg-small", action="store_true",
+                    help="short run for rocprofv3: --steps FP64 CG solves at 8^4 (kernel by kernel)")
+    ap.add_argument("--profile-block", action="store_true",
+                    help="short run for rocprofv3: the 24-rhs 32^4 configuration by block CG, at most "
+                         "--steps iterations (0: to 1e-10)")
     ap.add_argument("--profile-mre", action="store_true",
                     help="short run for rocprofv3: MRE guesses at 32^4 from 4 and 8 vectors")
     ap.add_argument("--profile-force", action="store_true",
@@ -805,6 +851,12 @@ def main(argv=None):
         return
     if a.profile_force:
         profile_force(W, a.steps)
+        return
+    if a.profile_cg_small:
+        profile_cg_small(W, a.steps)
+        return
+    if a.profile_block:
+        profile_block(W, a.steps)
         return
 
     t_setup = time.perf_counter()

@@ -9,10 +9,11 @@
 //            Q S = Q  (orthonormalised: Q <- Q S^-1),  D <- Q + D S^H,  C <- S C
 //   ||R_j|| = ||C e_j||.
 //
-// Orthonormalisation is SVQB on the Gram matrix (eigendecomposition, wl_small.h), which drops
-// directions whose Gram eigenvalue falls below 1e-12 of the largest: dependent or converged
-// directions (a repeated source, a zero source) leave the block instead of breaking the
-// recurrence, and M is the matching thresholded pseudo-inverse.  The N x N algebra runs on
+// Orthonormalisation is Cholesky QR on the Gram matrix when it is well conditioned, else SVQB
+// (eigendecomposition, wl_small.h), which drops directions whose Gram eigenvalue falls below
+// 1e-12 of the largest: dependent or converged directions (a repeated source, a zero source)
+// leave the block instead of breaking the recurrence; M is the Cholesky inverse or the matching
+// thresholded pseudo-inverse.  The N x N algebra runs on
 // the host in FP64 after each of the two Gram reductions of an iteration (2 x N^2 complex
 // numbers each way); everything on the lattice runs on the device.
 //
@@ -392,7 +393,10 @@ int cg_block(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec, wl_cg
     B.dmat = (cx<double> *)c->staging;
     B.dgram = B.dmat + (size_t)4 * N * N;
     hipStream_t S = c->stream;
-    const double tol = spec.accuracy, tiny = 1e-12;
+    // tiny: eigenvalues below it (relative) are dependent directions, dropped from the block;
+    // chol_tol: Cholesky pivots must clear it (relative) for the fast path, else the
+    // eigendecomposition decides what to drop
+    const double tol = spec.accuracy, tiny = 1e-12, chol_tol = 1e-10;
 
     // source norms; zero sources give x = 0 and count as converged (HILA would divide 0 by 0)
     Mat G;
@@ -456,7 +460,7 @@ int cg_block(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec, wl_cg
         Mat H;
         if ((rc = B.gram(p, B.D->data, B.Z->data, H))) return rc;
         Mat M;
-        small::pinv_herm(N, H, tiny, M);
+        small::inverse_herm(N, H, chol_tol, M);
         Mat Coef = small::mul(N, M, C), mM = M;
         for (auto &v : mM) v = -v;
         if ((rc = B.upload({&Coef, &mM}))) return rc;
@@ -470,7 +474,7 @@ int cg_block(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec, wl_cg
         // Q S = Q; D = Q + D S^H; C = S C
         Mat Gq;
         if ((rc = B.gram(p, B.Q->data, B.Q->data, Gq))) return rc;
-        small::svqb(N, Gq, tiny, Sm, Sinv);
+        small::orthonormaliser(N, Gq, chol_tol, Sm, Sinv);
         Mat SH = small::adj(N, Sm);
         C = small::mul(N, Sm, C);
         if ((rc = B.upload({&Sinv, &SH}))) return rc;

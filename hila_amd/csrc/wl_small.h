@@ -1,6 +1,7 @@
 // Dense algebra on small complex matrices (the N x N blocks of block CG, N <= 64), on the host
-// in FP64: Hermitian eigendecomposition by cyclic Jacobi rotations, the SVQB orthonormaliser
-// and a thresholded pseudo-inverse.  Matrices are row-major n x n arrays of complex<double>.
+// in FP64: Hermitian eigendecomposition by cyclic Jacobi rotations, the SVQB orthonormaliser, a
+// thresholded pseudo-inverse, and their Cholesky fast paths for well-conditioned matrices.
+// Matrices are row-major n x n arrays of complex<double>.
 // Header-only, host C++ only (no HIP), so that the CPU tests compile it on their own.
 #pragma once
 
@@ -15,6 +16,11 @@ namespace small {
 using cplx = std::complex<double>;
 using Mat = std::vector<cplx>;  // row-major n x n
 
+// a b without the C99 special-value handling of std::complex multiplication
+inline cplx cmul(cplx a, cplx b) {
+    return {a.real() * b.real() - a.imag() * b.imag(), a.real() * b.imag() + a.imag() * b.real()};
+}
+
 inline Mat identity(int n) {
     Mat I((size_t)n * n, cplx(0));
     for (int i = 0; i < n; ++i) I[(size_t)i * n + i] = 1.0;
@@ -28,7 +34,7 @@ inline Mat mul(int n, const Mat &A, const Mat &B) {
         for (int k = 0; k < n; ++k) {
             const cplx a = A[(size_t)i * n + k];
             if (a == cplx(0)) continue;
-            for (int j = 0; j < n; ++j) C[(size_t)i * n + j] += a * B[(size_t)k * n + j];
+            for (int j = 0; j < n; ++j) C[(size_t)i * n + j] += cmul(a, B[(size_t)k * n + j]);
         }
     return C;
 }
@@ -148,6 +154,72 @@ inline int pinv_herm(int n, const Mat &H, double tol, Mat &M) {
         }
     }
     return kept;
+}
+
+// Cholesky factor of a Hermitian positive definite G: G = R^H R, R upper triangular.  Fails
+// (returns false) when a pivot is not above tol times the largest diagonal entry: G is then
+// (numerically) semi-definite and the caller takes the eigendecomposition instead.
+inline bool chol_upper(int n, const Mat &G, double tol, Mat &R) {
+    double dmax = 0;
+    for (int i = 0; i < n; ++i) dmax = std::max(dmax, G[(size_t)i * n + i].real());
+    if (!(dmax > 0)) return false;
+    R.assign((size_t)n * n, cplx(0));
+    for (int i = 0; i < n; ++i) {
+        double d = G[(size_t)i * n + i].real();
+        for (int k = 0; k < i; ++k) d -= std::norm(R[(size_t)k * n + i]);
+        if (!(d > tol * dmax)) return false;
+        const double rii = std::sqrt(d);
+        R[(size_t)i * n + i] = rii;
+        for (int j = i + 1; j < n; ++j) {
+            cplx v = G[(size_t)i * n + j];
+            for (int k = 0; k < i; ++k) v -= cmul(std::conj(R[(size_t)k * n + i]), R[(size_t)k * n + j]);
+            R[(size_t)i * n + j] = v / rii;
+        }
+    }
+    return true;
+}
+
+// inverse of an upper triangular R with a nonzero diagonal
+inline Mat inv_upper(int n, const Mat &R) {
+    Mat X((size_t)n * n, cplx(0));
+    for (int j = 0; j < n; ++j) {
+        X[(size_t)j * n + j] = 1.0 / R[(size_t)j * n + j];
+        for (int i = j - 1; i >= 0; --i) {
+            cplx v = 0;
+            for (int k = i + 1; k <= j; ++k) v += cmul(R[(size_t)i * n + k], X[(size_t)k * n + j]);
+            X[(size_t)i * n + j] = -v / R[(size_t)i * n + i];
+        }
+    }
+    return X;
+}
+
+// Orthonormaliser from the Gram matrix G = W^H W: W = Q S, Q = W Sinv.  Cholesky QR (S = R,
+// upper triangular) when every pivot clears tol; otherwise SVQB, which drops dependent
+// directions.  Returns the number of directions kept.
+inline int orthonormaliser(int n, const Mat &G, double tol, Mat &S, Mat &Sinv) {
+    if (chol_upper(n, G, tol, S)) {
+        Sinv = inv_upper(n, S);
+        return n;
+    }
+    return svqb(n, G, tol, S, Sinv);
+}
+
+// Inverse of a Hermitian positive definite H through its Cholesky factor when every pivot
+// clears tol, else the thresholded pseudo-inverse.  Returns the rank used.
+inline int inverse_herm(int n, const Mat &H, double tol, Mat &M) {
+    Mat R;
+    if (chol_upper(n, H, tol, R)) {
+        const Mat X = inv_upper(n, R);  // H^-1 = X X^H
+        M.assign((size_t)n * n, cplx(0));
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                cplx v = 0;
+                for (int k = std::max(i, j); k < n; ++k) v += cmul(X[(size_t)i * n + k], std::conj(X[(size_t)j * n + k]));
+                M[(size_t)i * n + j] = v;
+            }
+        return n;
+    }
+    return pinv_herm(n, H, tol, M);
 }
 
 }  // namespace small

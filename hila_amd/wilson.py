@@ -42,7 +42,7 @@ EXPORTED_SYMBOLS = [
     "wl_gauge_download", "wl_config_header", "wl_config_read", "wl_config_write",
     "wl_hasenbusch_apply", "wl_cg_solve_ex", "wl_mre_guess", "wl_field_copy_rhs",
     "wl_fermion_force", "wl_momentum_upload", "wl_momentum_download", "wl_fermion_force_step",
-    "wl_hasenbusch2_force_step", "wl_cg_solve_noalloc",
+    "wl_hasenbusch2_force_step", "wl_cg_solve_noalloc", "wl_fp16_forms",
 ]
 
 
@@ -135,6 +135,7 @@ def lib():
         L.wl_meo_apply.argtypes = [vp, vp, vp, d, i]
         L.wl_cg_solve.argtypes = [vp, vp, vp, d, d, i, C.POINTER(CGStats)]
         L.wl_cg_solve_noalloc.argtypes = [vp, vp, vp, d, d, i, C.POINTER(CGStats)]
+        L.wl_fp16_forms.argtypes = [i, vp, vp, i, vp]
         L.wl_cg_solve_mixed.argtypes = [vp, vp, vp, d, d, i, i, d, i, C.POINTER(CGStats)]
         L.wl_set_option.argtypes = [vp, i, i]
         L.wl_get_option.argtypes = [vp, i, C.POINTER(i)]

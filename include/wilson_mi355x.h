@@ -411,6 +411,14 @@ int wl_autotune(wl_ctx *ctx, int prec, int nrhs, wl_tune_result *out);
 int wl_time_op(wl_ctx *ctx, int what, wl_field *out, const wl_field *in, double kappa, int sign,
                int warmup, int iters, double *ms_per_launch);
 
+/* The packed FP16 projection's 16 operand forms on device `device`, for testing them bit for
+ * bit.  No counterpart in HILA.  a[i], b[i] are complex numbers as two IEEE halves (re in the
+ * low 16 bits); out[f * n + i] (16 n entries) is the form f = 4 k + F of a + C b with C = +1,
+ * -1, +i, -i for k = 0..3, returned as (re, im), (im, re), (re, -im) or (im, -re) for F = 0..3.
+ * Each half of a result is one FP16 addition of the selected, possibly negated, input halves
+ * (the negation is applied to the operands, so -(x + y) is computed as (-x) + (-y)). */
+int wl_fp16_forms(int device, const uint32_t *a, const uint32_t *b, int n, uint32_t *out);
+
 /* ---- request server: continuous batching of CG solves --------------------------------- */
 
 /* A server on one device runs CG solves of independent requests (each with its own lattice

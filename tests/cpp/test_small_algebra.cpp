@@ -63,6 +63,22 @@ int main() {
         check(buf, maxabs(sub(mul(n, mul(n, adj(n, Sinv), G), Sinv), identity(n))), 1e-10);
         std::snprintf(buf, sizeof buf, "n=%2d  S Sinv - I", n);
         check(buf, maxabs(sub(mul(n, S, Sinv), identity(n))), 1e-10);
+        // Cholesky fast paths on the full-rank matrix
+        Mat Sc, Scinv, Mc;
+        std::snprintf(buf, sizeof buf, "n=%2d  Cholesky QR keeps every direction", n);
+        check(buf, orthonormaliser(n, G, 1e-10, Sc, Scinv) == n ? 0.0 : 1.0, 0.0);
+        std::snprintf(buf, sizeof buf, "n=%2d  Cholesky S^H S - G", n);
+        check(buf, maxabs(sub(mul(n, adj(n, Sc), Sc), G)) / scale, 1e-12);
+        std::snprintf(buf, sizeof buf, "n=%2d  Cholesky S Sinv - I", n);
+        check(buf, maxabs(sub(mul(n, Sc, Scinv), identity(n))), 1e-10);
+        bool upper = true;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) upper = upper && Sc[(size_t)i * n + j] == cplx(0);
+        std::snprintf(buf, sizeof buf, "n=%2d  Cholesky factor upper triangular", n);
+        check(buf, upper ? 0.0 : 1.0, 0.0);
+        inverse_herm(n, G, 1e-10, Mc);
+        std::snprintf(buf, sizeof buf, "n=%2d  Cholesky inverse: inv(G) G - I", n);
+        check(buf, maxabs(sub(mul(n, Mc, G), identity(n))), 1e-10);
         // pseudo-inverse of a full-rank matrix is its inverse
         Mat M;
         pinv_herm(n, G, 1e-14, M);
@@ -82,7 +98,12 @@ int main() {
             check(buf, kd == n - 1 ? 0.0 : 1.0, 0.0);
             std::snprintf(buf, sizeof buf, "n=%2d  rank-deficient S^H S - G", n);
             check(buf, maxabs(sub(mul(n, adj(n, S), S), Gd)) / sd, 1e-10);
+            // the fast paths fall back to the eigendecomposition and drop the same direction
+            std::snprintf(buf, sizeof buf, "n=%2d  orthonormaliser falls back, drops one", n);
+            check(buf, orthonormaliser(n, Gd, 1e-10, S, Sinv) == n - 1 ? 0.0 : 1.0, 0.0);
             Mat Md;
+            std::snprintf(buf, sizeof buf, "n=%2d  inverse falls back to the pseudo-inverse", n);
+            check(buf, inverse_herm(n, Gd, 1e-10, Md) == n - 1 ? 0.0 : 1.0, 0.0);
             pinv_herm(n, Gd, 1e-12, Md);
             std::snprintf(buf, sizeof buf, "n=%2d  rank-deficient G pinv(G) G - G", n);
             check(buf, maxabs(sub(mul(n, mul(n, Gd, Md), Gd), Gd)) / sd, 1e-9);
