@@ -130,11 +130,45 @@ void expm3(const cd X[9], cd out[9]) {
     std::memcpy(out, P, sizeof(P));
 }
 
+// SU<3>::random(nhits) (sun_matrix.h:211-240): from the identity, each hit multiplies rows
+// (r, q) = (0,1), (0,2), (1,2) in turn from the left by a random SU(2) matrix, and the result is
+// reunitarized after every 16th hit and at the end.  The SU(2) matrix is the normalized
+// Gaussian 4-vector v in the Pauli form [[v0 + i v3, v2 + i v1], [-v2 + i v1, v0 - i v3]]
+// (its N == 2 branch); (v0, v1) and (v2, v3) are the two Box-Muller pairs of one complex draw
+// each, as Matrix::gaussian_random fills a real 4-vector (matrix.h:1507-1514).
+constexpr int kSu2Hits = 16;
+
+void su3_by_su2_hits(uint64_t seed, int64_t lex, int mu, cd m[9]) {
+    for (int i = 0; i < 9; ++i) m[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    static const int pr[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    const uint64_t cbase = 1024 + (uint64_t)mu * 2 * 3 * kSu2Hits;  // own component range
+    for (int h = 1; h <= kSu2Hits; ++h) {
+        for (int k = 0; k < 3; ++k) {
+            const uint64_t c = cbase + 2 * (3 * (uint64_t)(h - 1) + k);
+            const cd g0 = cgauss(seed, lex, c), g1 = cgauss(seed, lex, c + 1);
+            double v[4] = {g0.real(), g0.imag(), g1.real(), g1.imag()};
+            const double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
+            for (double &x : v) x /= n;
+            const cd a(v[0], v[3]), b(v[2], v[1]), cc(-v[2], v[1]), d(v[0], -v[3]);
+            const int r = pr[k][0], q = pr[k][1];
+            for (int i = 0; i < 3; ++i) {  // rows r, q <- [[a, b], [cc, d]] (rows r, q)
+                const cd x = m[3 * r + i], y = m[3 * q + i];
+                m[3 * r + i] = a * x + b * y;
+                m[3 * q + i] = cc * x + d * y;
+            }
+        }
+        if (h % 16 == 0) reunitarize(m);
+    }
+    if (kSu2Hits % 16 != 0) reunitarize(m);
+}
+
 void gen_link(int kind, double eps, uint64_t seed, int64_t lex, int mu, double *dst) {
     cd m[9];
     const uint64_t cbase = (uint64_t)mu * 16;
     if (kind == 0) {
         for (int i = 0; i < 9; ++i) m[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    } else if (kind == 3) {
+        su3_by_su2_hits(seed, lex, mu, m);
     } else if (kind == 1) {
         // Haar on U(3): Gram-Schmidt of a complex Ginibre matrix (positive diagonal of R);
         // the determinant phase fix then maps it to Haar on SU(3).
@@ -162,7 +196,7 @@ void gen_link(int kind, double eps, uint64_t seed, int64_t lex, int mu, double *
 extern "C" int wl_gen_gauge(const int dims[4], int kind, double eps, uint64_t seed,
                             double *const U[4]) {
     Geo g(dims);
-    if (!g.ok || !U || kind < 0 || kind > 2) return WL_E_ARG;
+    if (!g.ok || !U || kind < 0 || kind > 3) return WL_E_ARG;
     for (int mu = 0; mu < 4; ++mu)
         if (!U[mu]) return WL_E_ARG;
     const int Lx = g.L[0], Ly = g.L[1], Lz = g.L[2];

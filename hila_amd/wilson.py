@@ -192,9 +192,10 @@ def _ptr(a):
 # ------------------------------------------------------------------ seeded inputs ------
 
 def gen_gauge(dims, kind="smooth", eps=0.3, seed=1):
-    """Gauge field in HILA's layout, (4, V, 18) float64.  kind: 'unit', 'haar' or 'smooth'
-    (exp(i eps H), H traceless Hermitian with unit-variance Gaussian entries)."""
-    kinds = {"unit": 0, "haar": 1, "smooth": 2}
+    """Gauge field in HILA's layout, (4, V, 18) float64.  kind: 'unit', 'haar', 'smooth'
+    (exp(i eps H), H traceless Hermitian with unit-variance Gaussian entries) or 'su2_hits'
+    (HILA's SU<3>::random(): 16 hits of random SU(2) subgroup matrices)."""
+    kinds = {"unit": 0, "haar": 1, "smooth": 2, "su2_hits": 3}
     V = int(np.prod(dims))
     U = np.empty((4, V, 18), dtype=np.float64)
     ptrs = (C.c_void_p * 4)(*[U[m].ctypes.data for m in range(4)])

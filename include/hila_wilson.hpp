@@ -148,7 +148,7 @@ class GaugeField {
         : fdir{MatrixField(lat), MatrixField(lat), MatrixField(lat), MatrixField(lat)}, lat_(lat) {}
     std::vector<double> &direction(int mu) { return fdir[mu].data(); }
     const std::vector<double> &direction(int mu) const { return fdir[mu].data(); }
-    // kind 0 unit, 1 Haar random, 2 exp(i eps H)
+    // kind 0 unit, 1 Haar random, 2 exp(i eps H), 3 SU<3>::random() (16 SU(2) subgroup hits)
     void random(int kind, double eps, uint64_t seed) {
         double *p[4] = {direction(0).data(), direction(1).data(), direction(2).data(), direction(3).data()};
         check(wl_gen_gauge(lat_.size().data(), kind, eps, seed, p), "wl_gen_gauge");

@@ -513,7 +513,9 @@ int wl_host_free(void *p);
 /* Gauge field written in HILA's layout into U[mu] (V x 18 doubles each).
  * kind 0: unit links; 1: Haar-random SU(3); 2: exp(i*eps*H), H traceless Hermitian with
  * unit-variance complex Gaussian entries, reunitarized as SU<N>::reunitarize
- * (libraries/datatypes/sun_matrix.h:137-197). */
+ * (libraries/datatypes/sun_matrix.h:137-197); 3: HILA's SU<3>::random() -- 16 hits of random
+ * SU(2) matrices on the three row pairs of the identity (sun_matrix.h:211-240).  eps is read
+ * by kind 2 only. */
 int wl_gen_gauge(const int dims[4], int kind, double eps, uint64_t seed, double *const U[4]);
 /* Gaussian Wilson-vector field: HILA's AoS layout, V x 24 doubles (full lattice, EVEN_SITES_
  * FIRST); sites of the parities in parity_mask (1 even, 2 odd, 3 all) get re, im ~ N(0,1)

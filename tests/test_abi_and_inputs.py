@@ -81,7 +81,7 @@ def W_complex(U):
     return (u[..., 0] + 1j * u[..., 1]).reshape(4, -1, 3, 3)
 
 
-@pytest.mark.parametrize("kind,eps", [("haar", 0.0), ("smooth", 0.3), ("smooth", 0.5)])
+@pytest.mark.parametrize("kind,eps", [("haar", 0.0), ("smooth", 0.3), ("smooth", 0.5), ("su2_hits", 0.0)])
 def test_links_are_su3(kind, eps):
     U = W.gen_gauge((8, 8, 8, 8), kind, eps, 1)
     unit, det = _su3_checks(U)
@@ -95,6 +95,27 @@ def test_haar_moments():
     n = tr.size
     assert abs(tr.mean()) < 6 / np.sqrt(n)
     assert abs((np.abs(tr) ** 2).mean() - 1) < 6 * np.sqrt(2.0 / n) * 1.5
+
+
+def test_su2_hits_links_are_haar_like():
+    """HILA's SU<3>::random() (16 hits of random SU(2) subgroup matrices on the three row
+    pairs) converges to the Haar measure: its links must show Haar SU(3)'s moments,
+    E[tr U] = 0, E[|tr U|^2] = 1, E[|tr U|^4] = 2 (k = 2 < N: the U(N) value k!) and
+    E[|U_ij|^2] = 1/3 for every entry.  A generator that mixed one row pair only leaves
+    U_22 = 1 and fails the first and last.  The moments pin the limit distribution, not the
+    sequence of hits: any SU(2) factors whose subgroups generate SU(3) tend to the Haar measure,
+    so the individual draws stay unpinned (HILA's random streams cannot be reproduced anyway)."""
+    m = W_complex(W.gen_gauge((8, 8, 8, 16), "su2_hits", 0, 11))
+    tr = np.trace(m, axis1=-2, axis2=-1).ravel()
+    n = tr.size
+    a2 = np.abs(tr) ** 2
+    assert abs(tr.mean()) < 6 / np.sqrt(n)
+    assert abs(a2.mean() - 1) < 6 * a2.std() / np.sqrt(n)
+    assert abs((a2 ** 2).mean() - 2) < 6 * (a2 ** 2).std() / np.sqrt(n)
+    e2 = (np.abs(m) ** 2).reshape(-1, 9)
+    assert np.all(np.abs(e2.mean(axis=0) - 1 / 3) < 6 * e2.std(axis=0) / np.sqrt(e2.shape[0]))
+    # not the same draws as the Ginibre-based Haar links of the same seed
+    assert not np.allclose(m, W_complex(W.gen_gauge((8, 8, 8, 16), "haar", 0, 11)))
 
 
 def test_smooth_links_distance_from_identity():
