@@ -401,7 +401,7 @@ def plain_f64_baseline(W, reqs, gauges, sources):
     return wall
 
 
-def serving_config(W, n_poisson=60, n_closed=24, rate=20.0, per_volume=3, seed=11):
+def serving_config(W, n_poisson=200, n_closed=24, rate=20.0, per_volume=3, seed=11):
     """Serving configuration: Poisson arrivals at `rate`/s of CG requests (volume from
     {16^4, 24^3x48, 32^4}, kappa ~ U[0.120, 0.1245], tolerance from {1e-8, 1e-10, 1e-12},
     1..12 Gaussian rhs, own gauge field exp(i eps H), eps ~ U[0.2, 0.5]) on the request server
@@ -417,6 +417,12 @@ def serving_config(W, n_poisson=60, n_closed=24, rate=20.0, per_volume=3, seed=1
     closed = draw_requests(rng, n_closed, per_volume)
     poisson = draw_requests(rng, n_poisson, per_volume)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_poisson))
+    # The client's page-locked solution buffers are allocated before any timing: a large
+    # hipHostMalloc blocks the HIP runtime, kernel launches of the server's threads included.
+    for d in SERVE_VOLUMES:
+        bufs = [pool.get(d) for _ in range(max(12, sum(1 for r in closed if r["dims"] == d)))]
+        for b in bufs:
+            pool.put(d, b)
     S = W.Server(max_active=16)
     # warm-up: one request per lattice size and precision (slots created, code paths loaded)
     warm = [{"dims": d, "g": 0, "kappa": 0.12, "tol": 1e-8, "nrhs": 12} for d in SERVE_VOLUMES]
@@ -476,7 +482,10 @@ def serving_config(W, n_poisson=60, n_closed=24, rate=20.0, per_volume=3, seed=1
         "rhs_solves_per_s": round(rhs_p / wall_p, 2),
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 1),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 1),
+        "p90_ms": round(float(np.percentile(lat_ms, 90)), 1),
         "mean_queue_ms": round(float(q_ms.mean()), 1),
+        "p99_queue_ms": round(float(np.percentile(q_ms, 99)), 1),
+        "arrival_span_s": round(float(arrivals[-1]), 2),
         "max_true_resid": float(max(tres)),
         "requests_meeting_tolerance": tol_ok,
         "precision": f"{best} inner (reliable-update delta {SERVE_PRECISIONS[best]}), FP64 "
@@ -503,14 +512,14 @@ def serving_config(W, n_poisson=60, n_closed=24, rate=20.0, per_volume=3, seed=1
 
 def profile_multi(W, steps):
     """Autotuned multi-rhs hopping launches (12 rhs at 32^4, links exp(i 0.3 H) seed 5,
-    Gaussian sources), FP16 then FP32 storage, then copies of a 12-rhs FP16 field (the byte
-    counters' calibration: 12 x 13 planes x stride x 4 B each way)."""
+    Gaussian sources), FP16, FP32 then FP64 storage, then copies of a 12-rhs FP16 field (the
+    byte counters' calibration: 12 x 13 planes x stride x 4 B each way)."""
     dims, N = (32, 32, 32, 32), 12
     lat = W.Lattice(dims)
     lat.upload_gauge(W.gen_gauge(dims, "smooth", 0.3, 5))
     D = W.Dirac_Wilson_evenodd(0.125, lat)
     src = W.gen_spinor(dims, 10, W.EVEN)
-    for prec in (W.F16, W.F32):
+    for prec in (W.F16, W.F32, W.F64):
         fin = W.Field(lat, W.EVEN, prec, nrhs=N)
         for j in range(N):
             fin.upload(src, rhs=j)
@@ -808,7 +817,7 @@ def main(argv=None):
                     help="skip writing and reading the 32^4 configuration file")
     ap.add_argument("--no-hmc", action="store_true",
                     help="skip the Hasenbusch CG and MRE-guess configuration")
-    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--profile-multi", action="store_true",
                     help="short run for rocprofv3: --steps autotuned multi-rhs hopping launches "
                          "(12 rhs, FP16 then FP32 storage) at 32^4, then --steps copies of one "
