@@ -502,6 +502,7 @@ extern "C" int wl_create(const int dims[4], int antiperiodic_t, int device, wl_c
     c->V = V;
     c->antiperiodic = antiperiodic_t ? 1 : 0;
     for (int i = 0; i < 4; ++i) c->g.L[i] = dims[i];
+    for (int i = 0; i < 3; ++i) c->g.dm[i] = div_multiplier((unsigned)dims[i]);
     c->g.vh = (int)(V / 2);
     c->g.stride = plane_stride(c->g.vh, -1);
     c->g.antiperiodic = c->antiperiodic;

@@ -297,7 +297,10 @@ __global__ __launch_bounds__(kBlock, 2) void k_dslash_rb(RbArgs<T> A) {
     using R = typename St<T>::R;
     using E = typename St<T>::E;
     if (A.all_done && *A.all_done) return;
-    const int W = xcd_tile(blockIdx.x, gridDim.x) * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    // the wave index, read from lane 0 so that the compiler knows it (and the segment, the rhs
+    // chunk and the rhs blocks' addresses) to be wave-uniform: scalar registers
+    const int W = xcd_tile(blockIdx.x, gridDim.x) * (int)(blockDim.x >> 6) +
+                  __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int o = W / A.C, chunk = W - o * A.C;
     const int k = segment_base(A.g, o) + (int)(threadIdx.x & 63);
     const int jf = A.j0 + chunk * NR;
@@ -313,7 +316,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_dslash_rb(RbArgs<T> A) {
 #pragma unroll
     for (int r = 0; r < NR; ++r) nrm[r] = 0;
     if (k < A.g.vh) {
-        const SiteNb n = neighbours(A.g, k, A.out_par);
+        const SiteNb n = neighbours<true>(A.g, k, A.out_par);
         const E *gout = A.gauge + (size_t)A.out_par * 4 * LF * S;
         const E *gin = A.gauge + (size_t)(1 - A.out_par) * 4 * LF * S;
         const R sf = (A.g.antiperiodic && n.fwd_edge_t) ? R(-1) : R(1);
@@ -419,7 +422,7 @@ __global__ __launch_bounds__(512, 1) void k_dslash_rb_lds(RbArgs<T> A) {
     for (int r = 0; r < NR; ++r) nrm[r] = 0;
     const bool live = k < A.g.vh;
     SiteNb n{};
-    if (live) n = neighbours(A.g, k, A.out_par);
+    if (live) n = neighbours<true>(A.g, k, A.out_par);
     const E *gout = A.gauge + (size_t)A.out_par * 4 * LF * S;
     const E *gin = A.gauge + (size_t)(1 - A.out_par) * 4 * LF * S;
     if (live) {
