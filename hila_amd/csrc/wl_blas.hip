@@ -171,6 +171,42 @@ __global__ __launch_bounds__(kBlock) void k_cg_xpay(cx<R> *__restrict__ p,
     site_xpay<R>(p, r, (R)st->beta, S, k);
 }
 
+// p = beta p + r of the launch-folded CG iteration `it` (0-based), with the finish kernel's
+// FIN_RRNEW folded in (conjugate_gradient.h:87-95): every workgroup adds the n partials of
+// ||r||^2 in the finish kernel's order, takes the same convergence decision and beta =
+// rrnew / rr_it, and updates p unless the solve has ended; workgroup 0 records the state
+// (rrnew, iter, done; beta and rr, with rr_(it+1) in slot (it + 1) & 1, unless converged).
+// The done flag is read once per workgroup: workgroup 0 may set it meanwhile, and p is not
+// needed once it is set.
+__global__ __launch_bounds__(kBlock) void k_cg_xpay_fold(cx<double> *__restrict__ p,
+                                                         const cx<double> *__restrict__ r, CgState *st,
+                                                         const double *parts, int n, int it, int vh, int S) {
+    __shared__ int ended;
+    if (threadIdx.x == 0) ended = st->done;
+    __syncthreads();
+    if (ended) return;  // workgroup-uniform
+    const double s = finish_order_sum(parts, n);
+    const double rr = st->rrb[it & 1];
+    const int iter = it + 1;
+    const bool conv = s < st->target_rr;
+    const double beta = s / rr;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rrnew = s;
+        st->iter = iter;
+        if (conv) {
+            st->done = 1;
+        } else {
+            st->beta = beta;
+            st->rr = s;
+            st->rrb[(it + 1) & 1] = s;
+            if (iter >= st->maxiters) st->done = 2;
+        }
+    }
+    if (conv || iter >= st->maxiters) return;
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < vh) site_xpay<double>(p, r, beta, S, k);
+}
+
 // Reliable update: r = b - DDp in FP64 (DDp = M^dag M x in FP64), stored in the inner
 // precision R, ||r||^2 reduced in FP64.
 template <typename R>
@@ -248,6 +284,13 @@ __global__ __launch_bounds__(kBlock) void k_finish(RedArgs ra, int n, int fin, c
 }
 
 // ------------------------------------------------------------------- launchers ---------
+
+hipError_t launch_cg_xpay_fold(const Geom &g, void *p, const void *r, CgState *st, const double *parts, int n,
+                               int it, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_xpay_fold, dim3(nblocks(g)), dim3(kBlock), 0, s, (cx<double> *)p, (const cx<double> *)r,
+                       st, parts, n, it, g.vh, g.stride);
+    return hipGetLastError();
+}
 
 hipError_t launch_finish(const RedArgs &ra, int nparts, int fin, const int *done, hipStream_t s) {
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(kBlock), 0, s, ra, nparts, fin, done);

@@ -18,7 +18,21 @@ namespace wl {
 
 namespace {
 
-// The kernel-by-kernel FP64 CG on work memory w.
+// Launch folding of the FP64 CG on small lattices: a hopping pass of at most this many
+// workgroups leaves its partial sums to the next kernel, which adds them itself in the finish
+// kernel's order, instead of a one-workgroup finish kernel between them.
+constexpr int kFoldParts = 512;
+
+bool cg_fold(const wl_ctx *c, const CgWork &w) {
+    const int nhop = hop_blocks(c->g);
+    return c->cg_persistent == 0 && nhop <= kFoldParts && w.partials_cap >= (size_t)2 * nhop;
+}
+
+// The kernel-by-kernel FP64 CG on work memory w.  An iteration is M p with ||Dp||^2 partials,
+// the finish of alpha, M^dag Dp fused with the x, r update and ||r||^2 partials, the finish of
+// rrnew (convergence, beta) and p = beta p + r: seven launches.  Folded (cg_fold), the alpha
+// finish moves into the update pass and the rrnew finish into p = beta p + r: five launches,
+// the same sums in the same order, so the same bits.
 int cg_f64(wl_ctx *c, const CgWork &w, void *x, size_t x_bytes, const void *b, double kappa, double accuracy,
            int maxiters, wl_cg_stats *stats) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -26,6 +40,9 @@ int cg_f64(wl_ctx *c, const CgWork &w, void *x, size_t x_bytes, const void *b, d
     void *r = w.r, *p = w.p, *Dp = w.Dp, *DDp = w.DDp, *t = w.t;
     const RedArgs ra = red_args(w);
     const int *done = &w.st->done;
+    const bool fold = cg_fold(c, w);
+    const int nhop = hop_blocks(c->g);
+    double *parts2 = w.partials + nhop;  // folded: partials of the update pass
 
     CgState init{};
     init.acc2 = accuracy * accuracy;
@@ -62,8 +79,8 @@ int cg_f64(wl_ctx *c, const CgWork &w, void *x, size_t x_bytes, const void *b, d
     while (!done_h && enqueued < maxiters) {
         const int n = std::min(kChunk, maxiters - enqueued);
         for (int it = 0; it < n; ++it) {
-            // Dp = M p, pDp = |Dp|^2, alpha = rr / pDp
-            if ((rc = meo_one(c, ra, WL_F64, Dp, p, t, kappa, +1, EPI_AXPY_NORM, done, FIN_ALPHA)))
+            // Dp = M p, pDp = |Dp|^2, alpha = rr / pDp (folded: alpha in the update pass)
+            if ((rc = meo_one(c, ra, WL_F64, Dp, p, t, kappa, +1, EPI_AXPY_NORM, done, fold ? FIN_NONE : FIN_ALPHA)))
                 return rc;
             // DDp = M^dag Dp fused with x += alpha p, r -= alpha DDp, rrnew = |r|^2, test, beta
             if ((rc = dslash_one(c, ra, WL_F64, t, 1, Dp, nullptr, -kappa, -1, EPI_SET, done, FIN_NONE)))
@@ -86,9 +103,19 @@ int cg_f64(wl_ctx *c, const CgWork &w, void *x, size_t x_bytes, const void *b, d
             L.r = r;
             L.red = ra;
             L.fin = FIN_RRNEW;
+            if (fold) {
+                L.red.partials = parts2;
+                L.fin = FIN_NONE;
+                L.fold_parts = w.partials;
+                L.fold_n = nhop;
+                L.fold_it = enqueued + it;
+            }
             HIPCHK(launch_dslash(WL_F64, c->g, L, c->stream));
-            // p = beta p + r
-            HIPCHK(launch_cg_xpay(WL_F64, c->g, p, r, w.st, c->stream));
+            // p = beta p + r (folded: with the convergence test and beta)
+            if (fold)
+                HIPCHK(launch_cg_xpay_fold(c->g, p, r, w.st, parts2, nhop, enqueued + it, c->stream));
+            else
+                HIPCHK(launch_cg_xpay(WL_F64, c->g, p, r, w.st, c->stream));
         }
         enqueued += n;
         HIPCHK(hipMemcpyAsync(&c->st_host->done, &w.st->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -556,7 +583,9 @@ int cg_batched(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec, wl_
 }  // namespace
 
 size_t cg_partials_needed(const wl_ctx *c) {
-    return (size_t)std::max({(c->g.vh + c->g.blk - 1) / c->g.blk, hop_blocks(c->g), nblocks(c->g)});
+    const int nhop = hop_blocks(c->g);
+    return (size_t)std::max({(c->g.vh + c->g.blk - 1) / c->g.blk, nhop, nblocks(c->g),
+                             nhop <= kFoldParts ? 2 * nhop : 0});  // two regions when folded
 }
 
 int cg_work_of(wl_ctx *c, CgWork *w) {

@@ -69,25 +69,6 @@ __device__ bool grid_sync(GridBar *bar) {
     return ok != 0;
 }
 
-// The sum that block_sum gives over 256 lanes holding f(0..255), computed by a workgroup of
-// any multiple of 64 lanes (valid in thread 0): each 64-lane group is reduced by the same
-// wave_sum, and the 4 group sums are added in order.
-template <typename F>
-__device__ double vthreads256_sum(F f) {
-    __shared__ double part[kBlock / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int vw = w; vw < kBlock / 64; vw += nw) {
-        const double v = wave_sum(f(vw * 64 + lane));
-        if (lane == 0) part[vw] = v;
-    }
-    __syncthreads();
-    double s = 0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < kBlock / 64; ++i) s += part[i];
-    __syncthreads();
-    return s;
-}
-
 struct CgpArgs {
     Geom g;                    // the hopping passes' geometry; g.blk = blockDim.x
     const cx<double> *gauge;   // FP64 links (two-row copy when LF = 6)
@@ -142,10 +123,13 @@ __device__ void hop_pass(const CgpArgs &P, cx<double> *out, int out_par, const c
     A.red.st = sst;
     A.red.out = nullptr;
     A.fin = FIN_NONE;
+    A.fold_parts = nullptr;
+    A.fold_n = 0;
+    A.fold_it = 0;
     for (int vb = blockIdx.x; vb < P.nbh; vb += gridDim.x) {
         const int k = lane_site_of(P.g, vb, P.nbh);
         double nrm = 0;
-        if (k < P.g.vh) nrm = dslash_site<double, SIGN, EPI, LF>(A, k);
+        if (k < P.g.vh) nrm = dslash_site<double, SIGN, EPI, LF>(A, k, sst->alpha);
         if constexpr (EPI == EPI_AXPY_NORM || EPI == EPI_CGUPD) {
             const double bs = block_sum(nrm);
             if (threadIdx.x == 0) P.partials[vb] = bs;

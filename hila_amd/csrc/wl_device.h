@@ -386,6 +386,10 @@ __device__ __forceinline__ double block_sum(double v) {
 struct CgState {
     double source_norm, target_rr, rr, pDp, rrnew, alpha, beta, acc2;
     double true_rr, scratch;
+    double rrb[2];      // rr by iteration parity (launch-folded single-rhs CG): slot i & 1 holds
+                        // the rr iteration i starts from, so that the pass that finishes
+                        // iteration i can record rr_(i+1) while its other workgroups still read
+                        // rr_i
     double rr_update;   // ||r||^2 at the last reliable update
     double delta2;      // update when ||r||^2 < delta2 * rr_update
     int iter, done, maxiters, nreliable;
@@ -423,7 +427,10 @@ __device__ __forceinline__ void fin_apply(int fin, double s, const RedArgs &ra) 
         st->target_rr = st->acc2 * s;
         if (s == 0.0) st->done = 1;  // b = 0: x = 0 is the solution
         break;
-    case FIN_RR: st->rr = s; break;
+    case FIN_RR:
+        st->rr = s;
+        st->rrb[0] = s;
+        break;
     case FIN_ALPHA:
         st->pDp = s;
         st->alpha = st->rr / s;
@@ -481,6 +488,40 @@ __device__ __forceinline__ void fin_apply(int fin, double s, const RedArgs &ra) 
         break;
     default: break;
     }
+}
+
+// The sum that block_sum gives over 256 lanes holding f(0..255), computed by a workgroup of
+// any multiple of 64 lanes (valid in thread 0): each 64-lane group is reduced by the same
+// wave_sum, and the 4 group sums are added in order.  Contains barriers.
+template <typename F>
+__device__ double vthreads256_sum(F f) {
+    __shared__ double part[kBlock / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int vw = w; vw < kBlock / 64; vw += nw) {
+        const double v = wave_sum(f(vw * 64 + lane));
+        if (lane == 0) part[vw] = v;
+    }
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kBlock / 64; ++i) s += part[i];
+    __syncthreads();
+    return s;
+}
+
+// The sum of n partials exactly as the one-workgroup finish kernel k_finish adds them (256
+// lanes, lane v adding partials v, v + 256, ...; then block_sum), returned to every thread of
+// a workgroup of any multiple of 64 lanes.  Contains barriers.
+__device__ __forceinline__ double finish_order_sum(const double *partials, int n) {
+    __shared__ double out;
+    const double s = vthreads256_sum([&](int v) {
+        double a = 0;
+        for (int i = v; i < n; i += kBlock) a += partials[i];
+        return a;
+    });
+    if (threadIdx.x == 0) out = s;
+    __syncthreads();
+    return out;
 }
 
 // Each workgroup of a reducing kernel stores its partial sum in partials[blockIdx.x]; a

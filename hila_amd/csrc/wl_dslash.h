@@ -25,13 +25,34 @@ struct DslashArgs {
     cx<R> *__restrict__ r;
     RedArgs red;
     int fin;
+    // EPI_CGUPD in the launch-folded CG: alpha = rr / (the sum of fold_n partials of the norm
+    // pass before, in the finish kernel's order), rr from red.st->rrb[fold_it & 1]; null: alpha
+    // from red.st (set by the finish kernel)
+    const double *fold_parts;
+    int fold_n;
+    int fold_it;
 };
+
+// alpha of the EPI_CGUPD epilogue (conjugate_gradient.h:79-80).  Folded: every workgroup adds
+// the previous pass's partials in the finish kernel's order, so all get FIN_ALPHA's bits, and
+// workgroup 0 records pDp and alpha.  Contains barriers when folded: every thread calls it.
+template <typename R>
+__device__ __forceinline__ double cg_alpha(const DslashArgs<R> &A) {
+    if (!A.fold_parts) return A.red.st->alpha;
+    const double s = finish_order_sum(A.fold_parts, A.fold_n);
+    const double alpha = A.red.st->rrb[A.fold_it & 1] / s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        A.red.st->pDp = s;
+        A.red.st->alpha = alpha;
+    }
+    return alpha;
+}
 
 // Output site k (< vh) of H_SIGN with epilogue EPI; returns the site's contribution to the
 // norm the epilogue reduces (0 for the others).  LF: stored entries per link, 9 or 6
-// (two-row links, wl_store.h).
+// (two-row links, wl_store.h).  alpha: the CG step of EPI_CGUPD.
 template <typename R, int SIGN, int EPI, int LF>
-__device__ __forceinline__ double dslash_site(const DslashArgs<R> &A, int k) {
+__device__ __forceinline__ double dslash_site(const DslashArgs<R> &A, int k, double alpha = 0) {
     const int S = A.g.stride;
     double nrm = 0;
     {
@@ -59,7 +80,6 @@ __device__ __forceinline__ double dslash_site(const DslashArgs<R> &A, int k) {
                 }
             st_site<R, true>(A.out, S, k, v);
         } else {  // EPI_CGUPD: x (FP64) += alpha p, r -= alpha DDp in R
-            const double alpha = A.red.st->alpha;
             const R alpha_r = (R)alpha;
             cx<R> ddp[4][3], pv[4][3], rv[4][3];
             ld_site<R, true>(A.xin, S, k, ddp);

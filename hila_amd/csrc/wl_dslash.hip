@@ -27,9 +27,11 @@ namespace wl {
 template <typename R, int SIGN, int EPI, int LF>
 __global__ __launch_bounds__(kBlock) void k_dslash(DslashArgs<R> A) {
     if (A.done && *A.done) return;
+    double alpha = 0;
+    if constexpr (EPI == EPI_CGUPD) alpha = cg_alpha(A);
     const int k = lane_site(A.g);
     double nrm = 0;
-    if (k < A.g.vh) nrm = dslash_site<R, SIGN, EPI, LF>(A, k);
+    if (k < A.g.vh) nrm = dslash_site<R, SIGN, EPI, LF>(A, k, alpha);
     if constexpr (EPI == EPI_AXPY_NORM || EPI == EPI_CGUPD) reduce_partial(nrm, A.red);
 }
 
@@ -59,6 +61,8 @@ __device__ __forceinline__ void dir_pair(cx<R> (&acc)[4][3], const DslashArgs<R>
 template <typename R, int SIGN, int EPI, int LF>
 __global__ __launch_bounds__(256) void k_dslash_dir(DslashArgs<R> A) {
     if (A.done && *A.done) return;
+    double alpha = 0;
+    if constexpr (EPI == EPI_CGUPD) alpha = cg_alpha(A);
     __shared__ cx<R> part[4][12][64];  // [direction][component][site]
     const int w = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
     const int k = xcd_tile(blockIdx.x, gridDim.x) * 64 + l;
@@ -120,7 +124,6 @@ __global__ __launch_bounds__(256) void k_dslash_dir(DslashArgs<R> A) {
                 stc<true>(A.out + e[c], v);
             }
         } else {  // EPI_CGUPD: x (FP64) += alpha p, r -= alpha DDp
-            const double alpha = A.red.st->alpha;
             const R alpha_r = (R)alpha;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -170,6 +173,9 @@ static hipError_t launch_t(const Geom &g, const DslashLaunch &L, hipStream_t s) 
     A.r = (cx<R> *)L.r;
     A.red = L.red;
     A.fin = L.fin;
+    A.fold_parts = L.fold_parts;
+    A.fold_n = L.fold_n;
+    A.fold_it = L.fold_it;
     const int nb = hop_blocks(g);
     if (g.dir) {
         if (L.lf == 6)
@@ -182,7 +188,8 @@ static hipError_t launch_t(const Geom &g, const DslashLaunch &L, hipStream_t s) 
         hipLaunchKernelGGL((k_dslash<R, SIGN, EPI, 9>), dim3(nb), dim3(g.blk), g.lds, s, A);
     }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess || (EPI != EPI_AXPY_NORM && EPI != EPI_CGUPD)) return e;
+    // FIN_NONE: the partials are left for the next kernel (launch-folded CG)
+    if (e != hipSuccess || (EPI != EPI_AXPY_NORM && EPI != EPI_CGUPD) || L.fin == FIN_NONE) return e;
     return launch_finish(L.red, nb, L.fin, L.done, s);
 }
 

@@ -50,7 +50,11 @@ struct DslashLaunch {
     const void *p;
     void *r;
     RedArgs red;
-    int fin;
+    int fin;               // applied by a finish kernel after a norm epilogue; FIN_NONE: none
+    // EPI_CGUPD of the launch-folded CG: alpha from fold_n partials (DslashArgs); null: none
+    const double *fold_parts = nullptr;
+    int fold_n = 0;
+    int fold_it = 0;
 };
 
 // Kernel launchers (wl_dslash.hip / wl_blas.hip).  All enqueue on `s` and return hipError_t.
@@ -90,6 +94,10 @@ hipError_t launch_cg_init(const Geom &g, const void *b, const void *DDp, void *r
 // p = beta p + r, beta from the CG state (conjugate_gradient.h:93-94); skipped when done != 0
 hipError_t launch_cg_xpay(int prec, const Geom &g, void *p, const void *r, const CgState *st,
                           hipStream_t s);
+// the same in FP64 for iteration `it` of the launch-folded CG, with the finish of the n partials
+// of ||r||^2 at `parts` folded in (convergence test, beta, state; k_cg_xpay_fold)
+hipError_t launch_cg_xpay_fold(const Geom &g, void *p, const void *r, CgState *st, const double *parts, int n,
+                               int it, hipStream_t s);
 // reliable update: r_lo = b - DDp (FP64 in, inner precision out), ||.||^2 -> FIN_UPDATE
 hipError_t launch_residual_update(int prec, const Geom &g, const void *b, const void *DDp,
                                   void *r_lo, const RedArgs &ra, hipStream_t s);
@@ -229,7 +237,8 @@ struct wl_ctx {
     bool use_graphs = false;           // batched CG: replay each chunk from a captured HIP graph
     int pipeline = 0;                  // multi-rhs hops: 0 pipelined where possible, 1 never
     int cg_persistent = 0;             // single-rhs FP64 CG in one launch: 1 yes (where the
-                                       // launch fits), 0 (default) or -1 kernel by kernel
+                                       // launch fits), 0 (default: kernel by kernel, launch-
+                                       // folded on small lattices) or -1 kernel by kernel
     void *grid_bar = nullptr;          // grid barrier of the persistent CG (on first use)
     bool block_krylov = false;         // WL_OPT_BLOCK_KRYLOV: several rhs share one Krylov space
     int small_hop = 0;                 // WL_OPT_SMALL_HOP (-1 never, 0 automatic, 1 always);

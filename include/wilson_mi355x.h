@@ -337,10 +337,13 @@ int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, d
  *   WL_OPT_CG_PERSISTENT  single-rhs FP64 CG (wl_cg_solve, wl_cg_solve_ex with FP64 inner):
  *       1 = the whole solve -- source norm, initial residual, every iteration and the final
  *       true residual -- runs as one cooperative launch whose workgroups meet at grid
- *       barriers, with no host round trip inside the solve; -1 = kernel by kernel, with the
- *       host checking convergence every 8 iterations; 0 (default) = kernel by kernel: on
- *       MI355X the grid barriers of the one-launch solve (an L2 write-back and invalidate per
- *       XCD each) cost more than the kernel boundaries they replace, at every lattice size.
+ *       barriers, with no host round trip inside the solve; -1 = kernel by kernel, seven
+ *       launches per iteration, with the host checking convergence every 8 iterations; 0
+ *       (default) = kernel by kernel, and on lattices of up to 2^15 sites per parity with
+ *       the two finish kernels of an iteration folded into the kernels after them (five
+ *       launches).  On MI355X the grid barriers of the one-launch solve (an L2 write-back and
+ *       invalidate per XCD each) cost more than the kernel boundaries they replace, at every
+ *       lattice size.
  *       Both give the same solution, bit for bit: the persistent launch runs the same per-site
  *       code and sums every reduction in the same order.  Falls back to kernel by kernel when
  *       the launch does not fit on the device at once.  The one-launch solve always runs one

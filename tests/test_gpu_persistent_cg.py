@@ -99,3 +99,72 @@ def test_persistent_cg_two_rows_shift_maxiters_and_zero_source():
     with pytest.raises(W.WilsonError):
         lat.set_option(W.OPT_CG_PERSISTENT, 2)
     lat.close()
+
+
+# The launch-folded kernel-by-kernel CG (WL_OPT_CG_PERSISTENT = 0, lattices up to 2^15 sites
+# per parity: the alpha and rrnew finish kernels folded into the update pass and into
+# p = beta p + r) against the unfolded one (-1), with the small-lattice kernel and with the
+# one-lane kernel (128-lane workgroups: the partials are added by 256 virtual lanes), where the
+# fold applies and where it does not (20^4).
+FOLD_CASES = [
+    ((8, 8, 8, 8), "haar", 0.0, 1, 0.12, True, 2),
+    ((16, 16, 16, 16), "smooth", 0.3, 3, 0.125, True, 4),
+    ((6, 6, 6, 10), "smooth", 0.3, 61, 0.12, True, 5),
+    ((16, 16, 16, 32), "smooth", 0.3, 5, 0.124, False, 7),
+    ((20, 20, 20, 20), "smooth", 0.3, 7, 0.124, True, 8),
+]
+
+
+def solve_mode(lat, op, b, mode, small_hop, tol=1e-10, maxiters=10000):
+    lat.set_option(W.OPT_SMALL_HOP, small_hop)
+    lat.set_option(W.OPT_CG_PERSISTENT, mode)
+    x = W.Field(lat)
+    t = time.perf_counter()
+    st = W.CG(op, tol, maxiters).apply(b, x)[0]
+    ms = 1e3 * (time.perf_counter() - t)
+    lat.set_option(W.OPT_CG_PERSISTENT, 0)
+    return x.download(), st, ms
+
+
+@pytest.mark.parametrize("case", FOLD_CASES, ids=lambda c: "x".join(map(str, c[0])))
+def test_launch_folded_cg_equals_unfolded(case):
+    dims, kind, eps, seed, kappa, ap, sseed = case
+    print("launch-folded CG", dims, flush=True)
+    lat = W.Lattice(dims, antiperiodic_t=ap)
+    lat.upload_gauge(W.gen_gauge(dims, kind, eps, seed))
+    D = W.Dirac_Wilson_evenodd(kappa, lat)
+    b = W.Field(lat).upload(W.gen_spinor(dims, sseed, EVEN))
+    for small_hop in (0, -1):
+        ref = solve_mode(lat, D, b, -1, small_hop)
+        got = solve_mode(lat, D, b, 0, small_hop)
+        print(f"  small_hop {small_hop}: {ref[1]['iterations']} it, unfolded {ref[2]:.2f} ms, "
+              f"folded {got[2]:.2f} ms", flush=True)
+        same(ref, got)
+        assert got[1]["converged"] == 1 and got[1]["true_resid"] <= 1e-10
+    lat.close()
+
+
+def test_launch_folded_cg_maxiters_shift_two_rows_and_zero_source():
+    dims = (8, 8, 8, 8)
+    lat = W.Lattice(dims)
+    lat.upload_gauge(W.gen_gauge(dims, "haar", 0.0, 1))
+    D = W.Dirac_Wilson_evenodd(0.12, lat)
+    b = W.Field(lat).upload(W.gen_spinor(dims, 2, EVEN))
+    # maxiters ends the solve (done = 2 set by p = beta p + r), also inside a chunk of 8
+    for maxiters in (5, 8, 13):
+        ref = solve_mode(lat, D, b, -1, 0, maxiters=maxiters)
+        got = solve_mode(lat, D, b, 0, 0, maxiters=maxiters)
+        same(ref, got)
+        assert got[1]["converged"] == 0 and got[1]["iterations"] == maxiters
+    # two-row links
+    lat.set_option(W.OPT_LINK_ROWS, 2)
+    same(solve_mode(lat, D, b, -1, 0), solve_mode(lat, D, b, 0, 0))
+    lat.set_option(W.OPT_LINK_ROWS, 3)
+    # the Hasenbusch operator M + mh
+    H = W.Hasenbusch_operator(D, 0.2)
+    same(solve_mode(lat, H, b, -1, 0), solve_mode(lat, H, b, 0, 0))
+    # b = 0: x = 0
+    z = W.Field(lat).zero()
+    x0, st0, _ = solve_mode(lat, D, z, 0, 0)
+    assert not np.any(x0) and st0["converged"] == 1
+    lat.close()
