@@ -423,7 +423,7 @@ def serving_config(W, n_poisson=200, n_closed=24, rate=20.0, per_volume=3, seed=
         bufs = [pool.get(d) for _ in range(max(12, sum(1 for r in closed if r["dims"] == d)))]
         for b in bufs:
             pool.put(d, b)
-    S = W.Server(max_active=16)
+    S = W.Server()  # the library's default concurrency (6 solves at once)
     # warm-up: one request per lattice size and precision (slots created, code paths loaded)
     warm = [{"dims": d, "g": 0, "kappa": 0.12, "tol": 1e-8, "nrhs": 12} for d in SERVE_VOLUMES]
     for prec in SERVE_PRECISIONS:
@@ -440,7 +440,7 @@ def serving_config(W, n_poisson=200, n_closed=24, rate=20.0, per_volume=3, seed=
     # the same closed batch on a server whose slots read two-row links (WL_OPT_LINK_ROWS = 2;
     # the request links are special unitary): the FP64 reliable updates and the FP32 inner
     # operator read a third less link data
-    S2 = W.Server(max_active=16)
+    S2 = W.Server()
     S2.set_option(W.OPT_LINK_ROWS, 2)
     for prec in SERVE_PRECISIONS:
         run_served(W, S2, warm, gauges, sources, pool, precision=prec)
@@ -623,7 +623,7 @@ def profile_serve(W):
     gauges, sources = serving_pool(W, rng, 3)
     pool = _SolutionPool(W)
     closed = draw_requests(rng, 24, 3)
-    with W.Server(max_active=16) as S:
+    with W.Server() as S:
         res, wall = run_served(W, S, closed, gauges, sources, pool, precision="f16")
     print(f"# profile-serve: {len(closed)} requests, {sum(r['nrhs'] for r in closed)} rhs in "
           f"{wall:.2f} s", file=sys.stderr, flush=True)

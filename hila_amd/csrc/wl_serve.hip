@@ -104,6 +104,14 @@ size_t slot_estimate(const int dims[4], int prec, int cap, int link_rows) {
     return bytes;
 }
 
+// Default concurrency, from the 200-request Poisson run at 20/s over {16^4, 24^3x48, 32^4}
+// (FP16 inner, two repeats, one box): at most 4 / 6 / 8 / 12 / 16 solves at once gave p50
+// 254-272 / 219-247 / 246-337 / 361-434 / 351-487 ms, p99 662-709 / 721-725 / 828-891 /
+// 1005-1083 / 1192-1499 ms, and the 24-request closed batch took 0.88-0.90 / 0.92-0.94 /
+// 0.94-1.03 / 0.99-1.03 / 1.02 s.  A request's multi-rhs launches already fill the GPU; more
+// solves at once only share its caches.
+constexpr int kDefaultMaxActive = 6;
+
 int slot_link_rows(const std::vector<std::pair<int, int>> &opts) {
     int rows = 3;
     for (const auto &o : opts)
@@ -119,7 +127,7 @@ using namespace wl;
 struct wl_server {
     int device = 0;
     size_t budget = 0;
-    int max_active = 16;
+    int max_active = kDefaultMaxActive;
 
     std::mutex mu;
     std::condition_variable cv_load;    // loader: new request / slot freed
@@ -521,7 +529,7 @@ extern "C" int wl_server_create(int device, size_t hbm_budget_bytes, int max_act
     if (!S) return fail(WL_E_OOM, "host allocation");
     S->device = device;
     S->budget = hbm_budget_bytes;
-    S->max_active = max_active > 0 ? max_active : 16;
+    S->max_active = max_active > 0 ? max_active : kDefaultMaxActive;
     try {
         S->loader = std::thread(loader_main, S);
         S->scheduler = std::thread(scheduler_main, S);

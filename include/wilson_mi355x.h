@@ -487,7 +487,8 @@ typedef struct wl_server_info {
     int64_t slots_released;
 } wl_server_info;
 
-/* hbm_budget_bytes = 0: half of the device memory free at creation.  max_active <= 0: 16. */
+/* hbm_budget_bytes = 0: half of the device memory free at creation.  max_active <= 0: 6 (on
+ * MI355X fewer solves at once gave both lower latency and more throughput than 8-16). */
 int wl_server_create(int device, size_t hbm_budget_bytes, int max_active, wl_server **out);
 /* Queue a request.  The descriptor is copied; the host buffers it points to (gauge, source,
  * solution) must stay valid and unmodified until the request completes.  Buffers from
