@@ -361,8 +361,10 @@ int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, d
  *       one Krylov space (the residual block is kept orthonormal, R = Q C; dependent or zero
  *       directions are dropped from the block), every rhs ends at its FP64 true residual, and
  *       every stats entry reports the block's iteration count; 0 (default) = an independent
- *       CG recurrence per rhs sharing the operator applications (batched CG).  The request
- *       server always solves batched. */
+ *       CG recurrence per rhs sharing the operator applications (batched CG).  With 1, a field
+ *       of more than 24 rhs is refused (WL_E_ARG) and an FP16 inner precision too
+ *       (WL_E_NOTIMPL); one rhs is solved as without it.  The request server always solves
+ *       batched. */
 enum {
     WL_OPT_TRAVERSAL_CHUNK = 1,
     WL_OPT_BLOCK = 2,

@@ -97,6 +97,13 @@ def test_block_cg_repeated_and_zero_sources_and_shift():
     # more rhs than one block holds
     lat.set_option(W.OPT_BLOCK_KRYLOV, 1)
     b = W.Field(lat, EVEN, W.F64, 25)
-    with pytest.raises(W.WilsonError):
+    with pytest.raises(W.WilsonError) as e:
         W.CG(D, 1e-10).apply(b, W.Field(lat, EVEN, W.F64, 25), raise_on_noconv=True)
+    assert e.value.code == W.WL_E_ARG
+    # block CG has no FP16 inner precision
+    b3 = W.Field(lat, EVEN, W.F64, 3)
+    with pytest.raises(W.WilsonError) as e:
+        W.CG(D, 1e-10, precision="f16").apply(b3, W.Field(lat, EVEN, W.F64, 3), raise_on_noconv=True)
+    assert e.value.code == W.WL_E_NOTIMPL
+    lat.set_option(W.OPT_BLOCK_KRYLOV, 0)
     lat.close()
