@@ -53,7 +53,7 @@ struct Request {
     wl_solve_result res{};
     std::vector<wl_cg_stats> stats;
     bool done = false;
-    int bypassed = 0;  // later requests admitted while this one waited at the head of the queue
+    int bypassed = 0;  // later requests admitted before this one (admission order)
 };
 
 struct Slot {
@@ -247,13 +247,30 @@ bool slot_matches(const Slot &s, const wl_solve_request &r) {
            s.antiperiodic == (r.antiperiodic_t ? 1 : 0) && s.prec == r.inner_prec && s.cap >= r.nrhs;
 }
 
-// Admission: requests are taken in submission order.  When the head of the queue has to wait
-// for device memory (its slot does not fit the budget even after idle slots are released),
-// a later request among the next kBypassWindow may go first if it can start at once -- an
-// idle slot of its lattice is free, or its new slot fits the budget without releasing any --
-// until the head has been passed kMaxBypass times; then the queue waits for the head.
+// Admission order.  When a solve slot is free, the next request is the one with the least
+// estimated work (request_work) among the first kOrderWindow waiting, earlier first on ties --
+// unless one of them has been passed over kMaxPassed times (a later request admitted before
+// it), which then goes next: the oldest such.  When the chosen request has to wait for device
+// memory (its slot does not fit the budget even after idle slots are released), another among
+// the next kBypassWindow may go first if it can start at once -- an idle slot of its lattice is
+// free, or its new slot fits the budget without releasing any -- until the chosen one has been
+// passed over kMaxPassed times; then the queue waits for it.  A request's solve does not depend
+// on when it is admitted, only its queue time does.
+constexpr bool kShortestFirst = true;
+constexpr size_t kOrderWindow = 16;
 constexpr int kBypassWindow = 8;
-constexpr int kMaxBypass = 4;
+constexpr int kMaxPassed = 8;
+
+// Estimated work of a request: sites x right-hand sides x log10(1 / tolerance) (CG iterations
+// grow with the digits asked for).
+double request_work(const wl_solve_request &r) {
+    const double V = (double)r.dims[0] * r.dims[1] * r.dims[2] * r.dims[3];
+    const double digits = std::log10(1.0 / std::max(r.tolerance, 1e-300));
+    return V * std::max(r.nrhs, 1) * std::max(digits, 1.0);
+}
+
+// The request to admit next (the queue is not empty; called with the server lock held).
+std::shared_ptr<Request> next_request(wl_server *S);
 
 // A slot for r that needs no memory beyond the budget: an idle slot of its lattice, or room
 // for a new one (*need > 0) without releasing anything.
@@ -265,6 +282,30 @@ bool admissible_now(wl_server *S, const wl_solve_request &r, Slot **slot, size_t
     if (*slot) return true;
     *need = slot_estimate(r.dims, r.inner_prec, slot_capacity(r.nrhs), slot_link_rows(S->slot_opts));
     return S->bytes_in_use + *need <= S->budget;
+}
+
+std::shared_ptr<Request> next_request(wl_server *S) {
+    const size_t n = std::min(S->pending.size(), kOrderWindow);
+    for (size_t i = 0; i < n; ++i)
+        if (S->pending[i]->bypassed >= kMaxPassed) return S->pending[i];
+    if (!kShortestFirst) return S->pending.front();
+    size_t best = 0;
+    double wbest = request_work(S->pending[0]->r);
+    for (size_t i = 1; i < n; ++i) {
+        const double w = request_work(S->pending[i]->r);
+        if (w < wbest) {
+            best = i;
+            wbest = w;
+        }
+    }
+    return S->pending[best];
+}
+
+// Removes q from the queue; every request ahead of it has been passed over once more.
+void take_from_queue(wl_server *S, const std::shared_ptr<Request> &q) {
+    auto it = std::find(S->pending.begin(), S->pending.end(), q);
+    for (auto j = S->pending.begin(); j != it; ++j) (*j)->bypassed += 1;
+    S->pending.erase(it);
 }
 
 void loader_main(wl_server *S) {
@@ -281,7 +322,7 @@ void loader_main(wl_server *S) {
                     (S->generation != blocked_at || S->pending.size() > blocked_qlen));
         });
         if (S->pending.empty()) return;  // stopping, queue drained
-        std::shared_ptr<Request> q = S->pending.front();
+        std::shared_ptr<Request> q = next_request(S);
         // an idle slot of this lattice (smallest sufficient capacity)
         Slot *slot = nullptr;
         for (auto &sp : S->slots)
@@ -311,7 +352,7 @@ void loader_main(wl_server *S) {
             }
             if (S->bytes_in_use + need > S->budget) {
                 if (S->active == 0) {  // can never fit
-                    S->pending.pop_front();
+                    S->pending.erase(std::find(S->pending.begin(), S->pending.end(), q));
                     lk.unlock();
                     char msg[160];
                     snprintf(msg, sizeof(msg), "a slot of %zu bytes exceeds the HBM budget of %zu bytes", need,
@@ -320,20 +361,20 @@ void loader_main(wl_server *S) {
                     lk.lock();
                     continue;
                 }
-                // the head waits for memory: a later request that can start now goes first
+                // q waits for memory: another request that can start now goes first
                 std::shared_ptr<Request> pass;
-                if (q->bypassed < kMaxBypass) {
+                if (q->bypassed < kMaxPassed) {
                     const size_t n = std::min(S->pending.size(), (size_t)kBypassWindow + 1);
-                    for (size_t i = 1; i < n && !pass; ++i)
-                        if (admissible_now(S, S->pending[i]->r, &slot, &need)) pass = S->pending[i];
+                    for (size_t i = 0; i < n && !pass; ++i)
+                        if (S->pending[i] != q && admissible_now(S, S->pending[i]->r, &slot, &need))
+                            pass = S->pending[i];
                 }
                 if (!pass) {
                     blocked_at = S->generation;
                     blocked_qlen = S->pending.size();
                     continue;
                 }
-                q->bypassed += 1;
-                q = pass;
+                q = pass;  // take_from_queue counts the pass-over of every request ahead
             } else {
                 slot = nullptr;  // a new slot for the head
             }
@@ -359,7 +400,7 @@ void loader_main(wl_server *S) {
             S->slots_created += 1;
         }
         blocked_at = ~(uint64_t)0;
-        S->pending.erase(std::find(S->pending.begin(), S->pending.end(), q));
+        take_from_queue(S, q);
         slot->busy = true;
         slot->req = q;
         S->active += 1;

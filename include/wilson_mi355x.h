@@ -437,13 +437,16 @@ int wl_fp16_forms(int device, const uint32_t *a, const uint32_t *b, int n, uint3
  * once) and the device memory of a slot for its lattice fits in the HBM budget.  Slots (a
  * context with the gauge field, the rhs/solution fields and the CG work fields of a lattice
  * size) stay resident after a request and are reused by the next request of that size; idle
- * slots of other sizes are released when the budget requires it.  Requests are admitted in
- * submission order, except that when the oldest waiting request needs memory that running
- * requests hold, one of the next 8 that can start at once (an idle slot of its lattice, or
- * room for a new slot without releasing any) goes first; the oldest is passed at most 4
- * times, then the queue waits for it.  The solves of admitted requests overlap on the device:
- * each slot has its own stream, and the host drives all of them without waiting inside a
- * solve. */
+ * slots of other sizes are released when the budget requires it.  Order: the next request
+ * admitted is the one with the least estimated work (sites x rhs x log10(1 / tolerance))
+ * among the first 16 waiting, the earlier one on ties; a request passed over 8 times (later
+ * requests admitted before it) goes next instead.  When the chosen request needs memory that
+ * running requests hold, another of the next 8 that can start at once (an idle slot of its
+ * lattice, or room for a new slot without releasing any) goes first, until the chosen one has
+ * been passed over 8 times; then the queue waits for it.  The order changes queue times only:
+ * a request's solution does not depend on it.  The solves of admitted requests overlap on the
+ * device: each slot has its own stream, and the host drives all of them without waiting
+ * inside a solve. */
 typedef struct wl_server wl_server;
 
 typedef struct wl_solve_request {

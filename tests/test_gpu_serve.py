@@ -118,6 +118,34 @@ def test_queueing_beyond_max_active_and_zero_source():
         assert not sol[1].any() and st[1]["converged"] == 1 and st[1]["iterations"] == 0
 
 
+def test_admission_least_work_first_with_aging():
+    """One solve at a time: while a long request runs, the waiting requests are admitted by
+    least estimated work (sites x rhs x digits of the tolerance), the earlier one first on ties,
+    and a request passed over 8 times goes next.  The order is read from the queue times: with
+    one solve at a time every admission waits for the completion before it."""
+    import time
+    blocker = Req((16, 16, 16, 16), 31, 12, 0.1245, 1e-12, "f64", 0.3)
+    mid = Req((8, 8, 8, 16), 32, 6, 0.123, 1e-10, "f32", 0.3)
+    small = Req((4, 4, 4, 4), 33, 1, 0.122, 1e-8, "f32", 0.3)
+    with W.Server(max_active=1) as S:
+        tb, _ = blocker.submit(S)
+        t0 = time.perf_counter()
+        while S.info()["active"] == 0:  # the blocker holds the one solve slot
+            assert time.perf_counter() - t0 < 30
+            time.sleep(0.001)
+        tm, _ = mid.submit(S)
+        ts = [small.submit(S)[0] for _ in range(10)]
+        rb = S.wait(tb)[0]
+        rm = S.wait(tm)[0]
+        rs = [S.wait(t)[0] for t in ts]
+    assert rb["status"] == rm["status"] == W.WL_OK and all(r["status"] == W.WL_OK for r in rs)
+    qm, qs = rm["queue_ms"], [r["queue_ms"] for r in rs]
+    # the small requests first, in submission order, until the mid-size one has been passed
+    # over 8 times; then it, then the last two small ones
+    assert qs[:8] == sorted(qs[:8]), qs
+    assert max(qs[:8]) < qm < min(qs[8:]), (qm, qs)
+
+
 def test_maxiters_reports_noconv():
     r = Req((8, 8, 8, 8), 9, 2, 0.1245, 1e-12, "f32", 0.3, maxiters=5)
     with W.Server() as S:
@@ -136,13 +164,14 @@ def test_hbm_budget_admission_and_slot_reuse():
         one = S.info()["bytes_in_use"]
     assert one > 0
     # room for one slot of either size at a time: requests run one after another, the idle
-    # slot of the other size is released, and results do not change
+    # slot of the other size is released, and results do not change (the least-work order
+    # runs the three 8^4 requests before the 8^3 x 10 one: one release at least)
     with W.Server(budget_bytes=int(1.5 * one)) as S:
         subs = [q.submit(S) for q in (r8, r8, r4, r8)]
         outs = [(S.wait(t), sol) for t, sol in subs]
         info = S.info()
     assert all(res["status"] == W.WL_OK for (res, _), _ in outs)
-    assert info["slots"] == 1 and info["slots_released"] >= 2
+    assert info["slots"] == 1 and info["slots_released"] >= 1
     assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][1], outs[3][1])
     # a budget below one slot: the request is refused, not queued forever
     with W.Server(budget_bytes=1 << 20) as S:
