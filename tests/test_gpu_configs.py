@@ -47,7 +47,7 @@ def say(*a):
 @pytest.fixture(scope="module")
 def c2():
     dims = (32, 32, 32, 32)
-    say("C2 setup", dims)
+    say("32^4 FP64 setup", dims)
     U = W.gen_gauge(dims, "smooth", 0.3, 3)
     lat = W.Lattice(dims, antiperiodic_t=True)
     lat.upload_gauge(U)
@@ -66,7 +66,7 @@ def test_c2_both_hopping_blocks_both_signs(c2):
     for psi, pin, pout in ((psi_e, EVEN, ODD), (psi_o, ODD, EVEN)):
         fin, fout = W.Field(lat, pin).upload(psi), W.Field(lat, pout)
         for sign in (1, -1):
-            say("C2 hop_set", pin, "->", pout, "sign", sign)
+            say("32^4 FP64 hop_set", pin, "->", pout, "sign", sign)
             D.hop_set(fin, fout, sign)
             err = rel(fout.to_full(), orc.hop_set(U, kappa, psi, pout, sign))
             assert err <= DSLASH_TOL, (pin, sign, err)
@@ -81,7 +81,7 @@ def test_c2_apply_and_dagger(c2):
     psi = W.gen_spinor(dims, 4, EVEN)
     fin, fout = W.Field(lat).upload(psi), W.Field(lat)
     for sign in (1, -1):
-        say("C2 meo sign", sign)
+        say("32^4 FP64 meo sign", sign)
         (D.apply if sign == 1 else D.dagger)(fin, fout)
         err = rel(fout.to_full(), orc.meo(U, kappa, psi, sign))
         assert err <= DSLASH_TOL, (sign, err)
@@ -103,12 +103,12 @@ def test_c2_cg_matches_oracle_cg(c2):
     kappa, tol = 0.125, 1e-10
     b = W.gen_spinor(dims, 4, EVEN)
     fb, fx = W.Field(lat).upload(b), W.Field(lat)
-    say("C2 GPU CG")
+    say("32^4 FP64 GPU CG")
     st = W.CG(W.Dirac_Wilson_evenodd(kappa, lat), tol, 10000).apply(fb, fx)[0]
     assert st["converged"] == 1 and st["true_resid"] <= tol, st
-    say("C2 oracle CG (about 10 s on the host)")
+    say("32^4 FP64 oracle CG (about 10 s on the host)")
     x_ref, steps, _ = orc.cg(U, kappa, b, accuracy=tol)
-    say("C2 GPU", st["iterations"], "it; oracle", steps + 1, "it")
+    say("32^4 FP64 GPU", st["iterations"], "it; oracle", steps + 1, "it")
     assert abs(st["iterations"] - (steps + 1)) <= 1
     x = fx.to_full()
     assert rel(x, x_ref) <= 1e-8
@@ -156,7 +156,7 @@ def test_golden_fixtures_on_the_gpu(path):
 @pytest.fixture(scope="module")
 def c3():
     dims = (48, 48, 48, 96)
-    say("C3 setup", dims)
+    say("48^3x96 mixed setup", dims)
     U = W.gen_gauge(dims, "smooth", 0.3, 6)
     lat = W.Lattice(dims, antiperiodic_t=True)
     lat.upload_gauge(U)
@@ -173,9 +173,9 @@ def test_c3_fp64_operator_against_oracle(c3):
     psi = W.gen_spinor(dims, 7, EVEN)
     fin, fout = W.Field(lat).upload(psi), W.Field(lat)
     D.apply(fin, fout)
-    say("C3 oracle M (a few seconds on the host)")
+    say("48^3x96 mixed oracle M (a few seconds on the host)")
     err = rel(fout.to_full(), Oracle(dims).meo(U, kappa, psi, 1))
-    say("C3 M rel err", err)
+    say("48^3x96 mixed M rel err", err)
     assert err <= DSLASH_TOL
 
 
@@ -191,7 +191,7 @@ def test_c3_mixed_cg_to_fp64_true_residual(c3, precision, delta):
     x64, xm = W.Field(lat), W.Field(lat)
     s64 = W.CG(D, tol, 10000).apply(b, x64)[0]
     sm = W.CG(D, tol, 10000, precision=precision, delta=delta).apply(b, xm)[0]
-    say("C3", precision, sm, "FP64:", s64)
+    say("48^3x96 mixed", precision, sm, "FP64:", s64)
     assert s64["converged"] == 1 and sm["converged"] == 1
     assert sm["true_resid"] <= tol and sm["reliable_updates"] >= 1
     Mx, MdMx = W.Field(lat), W.Field(lat)
@@ -201,7 +201,7 @@ def test_c3_mixed_cg_to_fp64_true_residual(c3, precision, delta):
     tr = float(np.linalg.norm(bh - MdMx.download()) / np.linalg.norm(bh))
     assert tr <= 1.05 * tol, tr
     d = rel(xm.download(), x64.download())
-    say("C3", precision, "solution vs FP64 CG", d)
+    say("48^3x96 mixed", precision, "solution vs FP64 CG", d)
     assert d <= 1e-8
 
 
@@ -222,7 +222,7 @@ def test_c4_batched_24_rhs():
     reliable updates, to 1e-10.  Every rhs meets the FP64 true residual; a point and a
     Gaussian rhs are checked with the oracle's M^dag M and against single-rhs FP64 CG."""
     dims, kappa, N, tol = (32, 32, 32, 32), 0.125, 24, 1e-10
-    say("C4 setup")
+    say("32^4 24-rhs setup")
     U = W.gen_gauge(dims, "smooth", 0.3, 5)
     lat = W.Lattice(dims)
     lat.upload_gauge(U)
@@ -238,7 +238,7 @@ def test_c4_batched_24_rhs():
     for j, s in enumerate(srcs):
         b.upload(s, rhs=j)
     x = W.Field(lat, EVEN, W.F64, nrhs=N)
-    say("C4 batched solve")
+    say("32^4 24-rhs batched solve")
     st = W.CG(D, tol, 10000, precision="f32", delta=1e-3, update_every=100).apply(b, x)
     assert len(st) == N
     for j, s in enumerate(st):
@@ -246,7 +246,7 @@ def test_c4_batched_24_rhs():
     orc = Oracle(dims)
     b1, x1 = W.Field(lat), W.Field(lat)
     for j in (0, 12):
-        say("C4 check rhs", j)
+        say("32^4 24-rhs check rhs", j)
         xj = x.to_full(j)
         r = srcs[j] - orc.meo(U, kappa, orc.meo(U, kappa, xj, 1), -1)
         tr = float(np.linalg.norm(r) / np.linalg.norm(srcs[j]))
@@ -257,12 +257,12 @@ def test_c4_batched_24_rhs():
         assert s64["converged"] == 1
         assert rel(x.download(j), x1.download()) <= 1e-8
     # the same 24 rhs by block CG (one shared Krylov space), FP32 inner, FP64 reliable updates
-    say("C4 block solve")
+    say("32^4 24-rhs block solve")
     lat.set_option(W.OPT_BLOCK_KRYLOV, 1)
     xb = W.Field(lat, EVEN, W.F64, nrhs=N)
     sb = W.CG(D, tol, 10000, precision="f32", delta=1e-3, update_every=100).apply(b, xb)
     lat.set_option(W.OPT_BLOCK_KRYLOV, 0)
-    say("C4 block", sb[0]["iterations"], "iterations, batched", max(q["iterations"] for q in st))
+    say("32^4 24-rhs block", sb[0]["iterations"], "iterations, batched", max(q["iterations"] for q in st))
     for j, q in enumerate(sb):
         assert q["converged"] == 1 and q["true_resid"] <= tol, (j, q)
         assert rel(xb.download(j), x.download(j)) <= 1e-8, j
@@ -316,7 +316,7 @@ def test_c5_served_mix_alone_and_together():
     reqs = [ServedRequest(d) for d in draw_requests(6)]
     alone = []
     for i, r in enumerate(reqs):
-        say("C5 alone", i, r.dims, r.prec, "nrhs", r.nrhs, "tol", r.tol)
+        say("served mix alone", i, r.dims, r.prec, "nrhs", r.nrhs, "tol", r.tol)
         with W.Server(max_active=1) as S:
             t, sol = r.submit(S)
             res, st = S.wait(t)
@@ -327,7 +327,7 @@ def test_c5_served_mix_alone_and_together():
         alone.append(sol)
     order = list(range(len(reqs))) * 2
     order = order[::2] + order[1::2][::-1]
-    say("C5 mix", order)
+    say("served mix mix", order)
     with W.Server(max_active=8) as S:
         sub = [(i,) + reqs[i].submit(S) for i in order]
         for i, t, sol in sub:
@@ -341,7 +341,7 @@ def test_c5_served_mix_alone_and_together():
         V = 2 * r.Vh
         lim = (20 if r.prec == "f64" else 1.05) * r.tol
         for j in sorted({0, r.nrhs - 1}):
-            say("C5 oracle check", i, j)
+            say("served mix oracle check", i, j)
             x, bb = np.zeros((V, 24)), np.zeros((V, 24))
             x[: r.Vh], bb[: r.Vh] = alone[i][j], r.src[j]
             y = orc.meo(r.U, r.kappa, orc.meo(r.U, r.kappa, x, 1), -1)
