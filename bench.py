@@ -457,14 +457,14 @@ def serving_config(W, n_poisson=200, n_closed=24, rate=20.0, per_volume=3, seed=
     best = min(closed_wall, key=closed_wall.get)
     wall_c = closed_wall[best]
     # The Poisson run: a server for the faster inner precision whose slots are provisioned
-    # before timing, as a deployed server holds its slots: for each lattice size, as many
-    # 12-rhs requests at once as the server runs solves at once, so that every size has that
-    # many slots that fit any request.  Slots the timed run still creates are reported.
+    # before timing (wl_server_provision), as a deployed server holds its slots: for each
+    # lattice size as many 12-rhs slots as the server runs solves at once, so that any request
+    # finds one.  Slots the timed run still creates are reported.
     S = W.Server()
     per_size = S.info()["max_active"]
     for d in SERVE_VOLUMES:
-        run_served(W, S, [{"dims": d, "g": 0, "kappa": 0.12, "tol": 1e-10, "nrhs": 12}] * per_size, gauges,
-                   sources, pool, precision=best)
+        S.provision(d, 12, per_size, precision=best)
+    run_served(W, S, warm, gauges, sources, pool, precision=best)  # code paths loaded
     created0 = S.info()["slots_created"]
     res_p, wall_p = run_served(W, S, poisson, gauges, sources, pool, arrivals=arrivals,
                                precision=best)

@@ -306,6 +306,32 @@ int BatchCG::prepare(wl_ctx *c, int N, int prec, bool mixed) {
     return ensure_staging(c, (size_t)c->g.vh * 24 * sizeof(double));
 }
 
+int BatchCG::reserve(wl_ctx *c, int N, int prec, bool mixed) {
+    int rc;
+    if (!c->gauge64) {  // as gauge_upload_async allocates it
+        if ((rc = dev_alloc(c, &c->gauge64, gauge_elems(c->g) * 16, "gauge field"))) return rc;
+        HIPCHK(hipMemsetAsync(c->gauge64, 0, gauge_elems(c->g) * 16, c->stream));
+    }
+    if (prec == WL_F32 && !c->gauge32)
+        if ((rc = dev_alloc(c, &c->gauge32, gauge_elems(c->g) * 8, "FP32 gauge copy"))) return rc;
+    if (prec == WL_F16 && !c->gauge16)
+        if ((rc = dev_alloc(c, &c->gauge16, gauge_elems(c->g) * 4, "FP16 gauge copy"))) return rc;
+    if ((rc = ensure_st_mr(c, N))) return rc;
+    wl_field *f;
+    if ((rc = work_field(c, W_R, prec, WL_EVEN, N, &f)) || (rc = work_field(c, W_P, prec, WL_EVEN, N, &f)) ||
+        (rc = work_field(c, W_DP, prec, WL_EVEN, N, &f)) || (rc = work_field(c, W_T, prec, WL_ODD, N, &f)) ||
+        (rc = work_field(c, W_DP64, WL_F64, WL_EVEN, N, &f)) ||
+        (rc = work_field(c, W_DDP64, WL_F64, WL_EVEN, N, &f)) ||
+        (rc = work_field(c, W_T64, WL_F64, WL_ODD, N, &f)))
+        return rc;
+    if (mixed && prec != WL_F64)
+        if ((rc = work_field(c, W_Y, WL_F32, WL_EVEN, N, &f))) return rc;
+    const size_t rows = (size_t)std::max(nblocks(c->g), mr_segments(c->g));
+    if ((rc = ensure_partials(c, (size_t)N * rows))) return rc;
+    // the gauge upload's staging (one direction of HILA's AoS links) is the larger one
+    return ensure_staging(c, std::max((size_t)c->g.vh * 24, (size_t)c->V * 18) * sizeof(double));
+}
+
 int BatchCG::get_event(hipEvent_t *ev) {
     if (!free_ev_.empty()) {
         *ev = free_ev_.back();

@@ -615,6 +615,49 @@ extern "C" int wl_server_submit(wl_server *S, const wl_solve_request *req, uint6
     return WL_OK;
 }
 
+extern "C" int wl_server_provision(wl_server *S, const int dims[4], int antiperiodic_t, int inner_prec, int nrhs,
+                                   int count, int *created) {
+    if (!S || !dims || !created) return fail(WL_E_ARG, "null argument");
+    *created = 0;
+    for (int i = 0; i < 4; ++i)
+        if (dims[i] < 2 || (dims[i] & 1))
+            return fail(WL_E_ARG, "lattice size %d in direction %d: must be even and >= 2", dims[i], i);
+    if (nrhs < 1 || nrhs > 64) return fail(WL_E_ARG, "nrhs %d out of range 1..64", nrhs);
+    if (inner_prec != WL_F64 && inner_prec != WL_F32 && inner_prec != WL_F16)
+        return fail(WL_E_NOTIMPL, "inner precision %d", inner_prec);
+    if (count < 0) return fail(WL_E_ARG, "count %d < 0", count);
+    wl_solve_request r{};
+    std::memcpy(r.dims, dims, sizeof(r.dims));
+    r.antiperiodic_t = antiperiodic_t;
+    r.inner_prec = inner_prec;
+    r.nrhs = nrhs;
+    const int cap = slot_capacity(nrhs);
+    // the whole call under the server lock: admissions wait until the slots exist
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop) return fail(WL_E_STATE, "server is shutting down");
+    S->submitted = true;  // the slots are made with the options set so far
+    const size_t need = slot_estimate(dims, inner_prec, cap, slot_link_rows(S->slot_opts));
+    for (int k = 0; k < count; ++k) {
+        if (S->bytes_in_use + need > S->budget) break;
+        std::unique_ptr<Slot> sp;
+        int rc = create_slot(S, r, cap, S->slot_opts, &sp);
+        if (!rc) rc = BatchCG::reserve(sp->ctx, cap, inner_prec, inner_prec != WL_F64);
+        if (!rc && hipStreamSynchronize(sp->ctx->stream) != hipSuccess)
+            rc = fail(WL_E_DEVICE, "provisioning a slot failed");
+        if (rc) {
+            if (sp) destroy_slot(sp.get());
+            return rc;
+        }
+        sp->last_used = Clock::now();
+        S->bytes_in_use += sp->ctx->dev_bytes;
+        S->slots.push_back(std::move(sp));
+        S->slots_created += 1;
+        *created += 1;
+    }
+    S->cv_load.notify_all();
+    return WL_OK;
+}
+
 extern "C" int wl_server_wait(wl_server *S, uint64_t ticket, int timeout_ms, wl_solve_result *res,
                               wl_cg_stats *per_rhs) {
     if (!S || !res) return fail(WL_E_ARG, "null argument");

@@ -52,6 +52,10 @@ class BatchCG {
     // Allocate the resident work memory of an N-rhs solve at inner precision prec (the only
     // allocations of a solve; a server calls this once per slot).
     static int prepare(wl_ctx *c, int N, int prec, bool mixed);
+    // The same allocations before any gauge field is uploaded: the FP64 links and the inner
+    // precision's copy are allocated but not filled (a server provisioning idle slots).  Full
+    // links only (a context with two-row links builds its copies at first use).
+    static int reserve(wl_ctx *c, int N, int prec, bool mixed);
     // Enqueue the start of a solve of M^dag M x = b (x: initial guess) on c's stream.
     int begin(wl_ctx *c, void *x, const void *b, int N, const CgSpec &spec);
     // Retire completed host checks and enqueue more work; never waits.  Sets *finished once the

@@ -42,7 +42,7 @@ EXPORTED_SYMBOLS = [
     "wl_gauge_download", "wl_config_header", "wl_config_read", "wl_config_write",
     "wl_hasenbusch_apply", "wl_cg_solve_ex", "wl_mre_guess", "wl_field_copy_rhs",
     "wl_fermion_force", "wl_momentum_upload", "wl_momentum_download", "wl_fermion_force_step",
-    "wl_hasenbusch2_force_step", "wl_cg_solve_noalloc", "wl_fp16_forms",
+    "wl_hasenbusch2_force_step", "wl_cg_solve_noalloc", "wl_fp16_forms", "wl_server_provision",
 ]
 
 
@@ -145,6 +145,7 @@ def lib():
         L.wl_gen_point.argtypes = [ip4, C.c_uint64, i, i, vp, C.POINTER(i64)]
         L.wl_server_create.argtypes = [i, C.c_size_t, i, C.POINTER(vp)]
         L.wl_server_submit.argtypes = [vp, C.POINTER(SolveRequest), C.POINTER(C.c_uint64)]
+        L.wl_server_provision.argtypes = [vp, ip4, i, i, i, i, C.POINTER(i)]
         L.wl_server_wait.argtypes = [vp, C.c_uint64, i, C.POINTER(SolveResult), C.POINTER(CGStats)]
         L.wl_server_info_get.argtypes = [vp, C.POINTER(ServerInfo)]
         L.wl_server_destroy.argtypes = [vp]
@@ -634,6 +635,16 @@ class Server:
     def set_option(self, option, value):
         """A kernel option (OPT_*) for every slot; only before the first submission."""
         check(lib().wl_server_set_option(self._h, int(option), int(value)))
+
+    def provision(self, dims, nrhs, count, precision="f32", antiperiodic_t=True):
+        """Create up to `count` idle slots, with all their device memory, for requests of this
+        lattice, inner precision and at most nrhs rhs, within the HBM budget (wl_server_provision).
+        Returns the number created."""
+        made = C.c_int(0)
+        check(lib().wl_server_provision(self._h, _dims(dims), 1 if antiperiodic_t else 0,
+                                        {"f64": F64, "f32": F32, "f16": F16}[precision], int(nrhs),
+                                        int(count), C.byref(made)))
+        return made.value
 
     def info(self):
         i = ServerInfo()

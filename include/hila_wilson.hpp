@@ -660,6 +660,14 @@ class Server {
         check(wl_server_submit(srv_, &r, &t), "wl_server_submit");
         return t;
     }
+    // Idle slots, with their device memory, for up to nrhs-rhs requests of one lattice and inner
+    // precision, made before traffic; returns how many the HBM budget allowed.
+    int provision(const std::array<int, 4> &dims, int inner_prec, int nrhs, int count, bool antiperiodic_t = true) {
+        int made = 0;
+        check(wl_server_provision(srv_, dims.data(), antiperiodic_t ? 1 : 0, inner_prec, nrhs, count, &made),
+              "wl_server_provision");
+        return made;
+    }
     // Blocks until the request finishes; per_rhs (if given) receives r.nrhs entries.
     wl_solve_result wait(uint64_t ticket, wl_cg_stats *per_rhs = nullptr) {
         wl_solve_result res{};

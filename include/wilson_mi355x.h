@@ -499,6 +499,16 @@ int wl_server_create(int device, size_t hbm_budget_bytes, int max_active, wl_ser
  * solution) must stay valid and unmodified until the request completes.  Buffers from
  * wl_host_alloc are copied by DMA without staging. */
 int wl_server_submit(wl_server *srv, const wl_solve_request *req, uint64_t *ticket);
+/* Provision idle slots before traffic (no counterpart in HILA): up to `count` slots for
+ * requests of lattice dims, the time boundary antiperiodic_t, inner precision inner_prec and
+ * at most nrhs right-hand sides, each with all the device memory such a request uses (gauge
+ * field and the inner precision's link copy, source and solution fields, CG work fields), so
+ * that the requests that later take them allocate nothing.  Stops early, with WL_OK, when the
+ * next slot would exceed the HBM budget; *created = the slots made.  Holds the server lock
+ * while it runs (admissions wait), and freezes the slot options as a submission does.  Slots
+ * with two-row links (WL_OPT_LINK_ROWS = 2) build their two-row copies at first use. */
+int wl_server_provision(wl_server *srv, const int dims[4], int antiperiodic_t, int inner_prec, int nrhs,
+                        int count, int *created);
 /* Wait up to timeout_ms (< 0: no limit, 0: poll) for a request.  On completion fills *res and,
  * if per_rhs is not null, its nrhs entries, releases the ticket and returns WL_OK (the
  * request's own outcome is res->status); WL_E_BUSY if not finished in time. */

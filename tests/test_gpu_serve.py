@@ -146,6 +146,33 @@ def test_admission_least_work_first_with_aging():
     assert max(qs[:8]) < qm < min(qs[8:]), (qm, qs)
 
 
+def test_provisioned_slots_serve_without_allocating():
+    """wl_server_provision: idle slots made before traffic hold all the device memory a
+    request of their shape uses.  Requests that take them create no slot and allocate nothing
+    (bytes in use unchanged), and their solutions are those of a server without provisioning,
+    bit for bit.  Provisioning stops at the HBM budget."""
+    r = Req((8, 8, 8, 8), 41, 3, 0.122, 1e-10, "f32", 0.3)
+    base = solve_alone(r)
+    with W.Server(max_active=2) as S:
+        assert S.provision(r.dims, 4, 2, precision="f32") == 2
+        i0 = S.info()
+        assert i0["slots"] == 2 and i0["slots_created"] == 2 and i0["bytes_in_use"] > 0
+        subs = [r.submit(S) for _ in range(2)]
+        outs = [(S.wait(t), sol) for t, sol in subs]
+        i1 = S.info()
+    assert i1["slots_created"] == 2 and i1["bytes_in_use"] == i0["bytes_in_use"]
+    for (res, st), sol in outs:
+        assert res["status"] == W.WL_OK
+        assert np.array_equal(sol, base[2])
+    one = i0["bytes_in_use"] // 2
+    with W.Server(budget_bytes=int(1.5 * one)) as S:
+        assert S.provision(r.dims, 4, 3, precision="f32") == 1
+    with W.Server() as S:
+        with pytest.raises(W.WilsonError) as e:
+            S.provision((8, 8, 8, 7), 4, 1)
+        assert e.value.code == W.WL_E_ARG
+
+
 def test_maxiters_reports_noconv():
     r = Req((8, 8, 8, 8), 9, 2, 0.1245, 1e-12, "f32", 0.3, maxiters=5)
     with W.Server() as S:
