@@ -406,6 +406,7 @@ int main() {
         Lattice lattice({16, 8, 8, 8});  // test_CG.cpp:57, NDIM == 4
         run(lattice, 0, "unit");
         run(lattice, 1, "Haar random");
+        run(lattice, 3, "SU<3>::random (16 SU(2) hits, as test_CG.cpp draws them)");
         run_single_precision(lattice);
         run_hasenbusch(lattice);
         run_force(lattice);
