@@ -157,6 +157,12 @@ def mixed_precision_config(W, n):
     us64 = 1e3 * D.time(b, fo64, "hop", 1, warmup=3, iters=20)
     us32 = 1e3 * D.time(fin32, fo32, "hop", 1, warmup=3, iters=20)
     del fo64, fo32, fin32
+    # device copy of the FP64 hop's byte count (two 4-rhs FP64 fields of 4.1 GB), the rate a
+    # stream of that size reaches with no cache help
+    ca, cb = W.Field(lat, W.EVEN, W.F64, 4), W.Field(lat, W.EVEN, W.F64, 4)
+    copy_gbs = 2 * 4 * 192 * lat.Vh / (1e-3 * D.time(ca, cb, "copy", warmup=3, iters=20)) / 1e9
+    ca.close()
+    cb.close()
     x64, x32, x16 = W.Field(lat), W.Field(lat), W.Field(lat)
     r64 = solve_series(W, D, b, x64, n, "f64")
     r32 = solve_series(W, D, b, x32, n, "f32")
@@ -169,6 +175,8 @@ def mixed_precision_config(W, n):
            "solution_rel_diff_vs_f64": rel_diff(x32.download(), x64.download()),
            "dslash_f64_us": round(us64, 1),
            "dslash_f64_hbm_frac": round(1536 * lat.Vh / (us64 * 1e-6) / 8e12, 4),
+           "copy_gbs": round(copy_gbs, 1),
+           "dslash_f64_frac_of_copy": round(1536 * lat.Vh / (us64 * 1e-6) / 1e9 / copy_gbs, 4),
            "dslash_f32_us": round(us32, 1),
            "dslash_f32_hbm_frac": round(768 * lat.Vh / (us32 * 1e-6) / 8e12, 4)}
     lat.close()

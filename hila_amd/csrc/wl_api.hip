@@ -458,8 +458,9 @@ int auto_chunk(const Geom &g) {
     return 0;
 }
 
-// Sites per component plane: the parity's sites rounded up to a workgroup, plus `pad` sites
-// (pad < 0: the default, default_plane_pad).
+// Block stride S in sites (a field's rhs block holds planes x S elements, site-tiled): the
+// parity's sites rounded up to a workgroup, plus `pad` sites (pad < 0: the default,
+// default_plane_pad).  Always a multiple of the 64-site tile.
 int plane_stride(int vh, int pad) {
     const int base = (vh + kBlock - 1) / kBlock * kBlock;
     if (pad < 0) pad = default_plane_pad(base);

@@ -55,7 +55,7 @@ struct Vec16 {
 };
 template <typename T>
 __device__ __forceinline__ Vec16<T> ld_vec16(const typename St<T>::E *base, int q, int S, int k) {
-    const auto raw = ld_vec<false, T>(base + (size_t)q * St<T>::pack * S + (size_t)St<T>::pack * k);
+    const auto raw = ld_vec<false, T>(base + comp_elem<T>(St<T>::pack * q, S, k));
     Vec16<T> v;
 #pragma unroll
     for (int h = 0; h < St<T>::pack; ++h) v.e[h] = widen(raw.e[h]);

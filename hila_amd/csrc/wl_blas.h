@@ -25,7 +25,7 @@ __device__ __forceinline__ double site_cg_init(const cx<double> *__restrict__ b,
     double s = 0;
 #pragma unroll
     for (int c = 0; c < 12; ++c) {
-        const int e = c * S + k;
+        const size_t e = comp_elem<double>(c, S, k);
         const cx<double> v = b[e] - ddp[e];
         r[e] = v;
         p[e] = v;
@@ -53,7 +53,7 @@ __device__ __forceinline__ double site_diff_norm2(const cx<double> *__restrict__
     double s = 0;
 #pragma unroll
     for (int c = 0; c < 12; ++c) {
-        const int e = c * S + k;
+        const size_t e = comp_elem<double>(c, S, k);
         s += norm2(b[e] - v[e]);
     }
     return s;

@@ -13,7 +13,7 @@ namespace wl {
 // ------------------------------------------------------------------- layout ------------
 
 // HILA AoS (V/2 sites x 24 doubles) -> right-hand side j of an N-rhs field in storage T:
-// component c of site k of rhs j at element (j * planes + c) * S + k.
+// component c of site k of rhs j at element j * planes * S + comp_elem(c, S, k).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_aos_to_soa(const double *__restrict__ aos,
                                                        typename St<T>::E *__restrict__ out, int vh,
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kBlock) void k_soa_to_aos(const typename St<T>::E *
 }
 
 // One direction of HILA's Field<SU<3,double>> (V sites x 18 doubles, even sites first) ->
-// gauge planes of both parities.
+// gauge blocks of both parities (site-tiled, wl_store.h).
 __global__ __launch_bounds__(kBlock) void k_gauge_to_soa(const double *__restrict__ aos,
                                                          cx<double> *__restrict__ gauge, int mu,
                                                          int vh, int S) {
@@ -57,30 +57,40 @@ __global__ __launch_bounds__(kBlock) void k_gauge_to_soa(const double *__restric
     const double *src = aos + (size_t)18 * s;
     cx<double> *dst = gauge + (size_t)((p * 4 + mu) * 9) * S;
 #pragma unroll
-    for (int e = 0; e < 9; ++e) dst[e * S + k] = {src[2 * e], src[2 * e + 1]};
+    for (int e = 0; e < 9; ++e) dst[link_entry<double>(e, S, k)] = {src[2 * e], src[2 * e + 1]};
 }
 
-// FP64 links -> FP32, in the packed FP32 link order of wl_store.h: element i = (block * 9 + e)
-// * S + k of the FP64 planes is entry e of site k of block (parity, direction).
+// Entry e and site k of element `rem` of an FP64 (unpacked) block of ne entries per site.
+__device__ __forceinline__ void tile_entry_site(size_t rem, int ne, int &e, int &k) {
+    const size_t tile = rem / ((size_t)ne * kTile);
+    const int r = (int)(rem - tile * ne * kTile);
+    e = r / kTile;
+    k = (int)tile * kTile + (r - e * kTile);
+}
+
+// FP64 links -> FP32, in the packed FP32 link order of wl_store.h: element i of the FP64 links
+// is entry e of site k of block i / 9S (parity, direction).
 __global__ __launch_bounds__(kBlock) void k_convert_f32(const cx<double> *__restrict__ in,
                                                         cx<float> *__restrict__ out, size_t n, int S) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const size_t blk = i / ((size_t)9 * S), rem = i - blk * 9 * S;
-    const int e = (int)(rem / S), k = (int)(rem - (size_t)e * S);
+    const size_t blk = i / ((size_t)9 * S);
+    int e, k;
+    tile_entry_site(i - blk * 9 * S, 9, e, k);
     const cx<double> v = in[i];
     out[blk * 9 * S + link_entry<float>(e, S, k)] = {(float)v.x, (float)v.y};
 }
 
 // FP64 links -> FP16 (SU(3) entries are bounded by 1: no scale), in the packed FP16 link
-// order of wl_store.h: element i = (block * 9 + e) * S + k of the FP64 planes is entry e of
-// site k of block (parity, direction).
+// order of wl_store.h: element i of the FP64 links is entry e of site k of block i / 9S
+// (parity, direction).
 __global__ __launch_bounds__(kBlock) void k_convert_h2(const cx<double> *__restrict__ in,
                                                        h2 *__restrict__ out, size_t n, int S) {
     const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const size_t blk = i / ((size_t)9 * S), rem = i - blk * 9 * S;
-    const int e = (int)(rem / S), k = (int)(rem - (size_t)e * S);
+    const size_t blk = i / ((size_t)9 * S);
+    int e, k;
+    tile_entry_site(i - blk * 9 * S, 9, e, k);
     const cx<double> v = in[i];
     h2 q;
     q.x = (_Float16)(float)v.x;
@@ -89,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_convert_h2(const cx<double> *__restr
 }
 
 // Two-row link copy for the hopping kernels (rows 0 and 1 of every link, in the 6S-element
-// blocks of ld_link_raw's LF = 6 layout) from the FP64 planes; out may be null (check only).
+// blocks of ld_link_raw's LF = 6 layout) from the FP64 links; out may be null (check only).
 // With dev non-null each workgroup also writes the largest |re|, |im| of
 // U[2][j] - conj(row 0 x row 1)_j over its links to dev[workgroup]: how far the stored third
 // row is from the one the two-row kernels rebuild.  Thread i: link block i / vh (parity,
@@ -101,14 +111,14 @@ __global__ __launch_bounds__(kBlock) void k_two_rows(const cx<double> *__restric
     double d = 0;
     if (i < (size_t)8 * vh) {
         const int blk = (int)(i / vh), k = (int)(i - (size_t)blk * vh);
-        const cx<double> *src = in + (size_t)blk * 9 * S + k;
+        const cx<double> *src = in + (size_t)blk * 9 * S;
         cx<double> u[9];
 #pragma unroll
-        for (int e = 0; e < 9; ++e) u[e] = src[(size_t)e * S];
+        for (int e = 0; e < 9; ++e) u[e] = src[link_entry<double>(e, S, k)];
         if (out) {
 #pragma unroll
             for (int e = 0; e < 6; ++e)
-                out[(size_t)blk * 6 * S + link_entry<R>(e, S, k)] = {(R)u[e].x, (R)u[e].y};
+                out[(size_t)blk * 6 * S + link_entry<R, 6>(e, S, k)] = {(R)u[e].x, (R)u[e].y};
         }
         if (dev) {
             cx<double> w[9];
@@ -220,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_update(const cx<double> *__
         cx<R> v[4][3];
 #pragma unroll
         for (int c = 0; c < 12; ++c) {
-            const int e = c * S + k;
+            const size_t e = comp_elem<double>(c, S, k);
             const cx<double> d = b[e] - ddp[e];
             v[c / 3][c % 3] = {(R)d.x, (R)d.y};
             s += norm2(d);

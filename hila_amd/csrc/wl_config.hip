@@ -21,6 +21,7 @@
 #include <sys/stat.h>
 
 #include "wl_host.h"
+#include "wl_store.h"
 
 namespace wl {
 
@@ -37,7 +38,7 @@ struct d2 {
 };
 
 // Lexicographic links of a file lattice `in` (lex index x + in0 (y + in1 (z + in2 t)), 18
-// doubles each) -> direction mu of the gauge planes; checkerboard site (p, k) takes the link
+// doubles each) -> direction mu of the gauge blocks; checkerboard site (p, k) takes the link
 // of its coordinates reduced modulo the file lattice.
 __global__ __launch_bounds__(kBlock) void k_gauge_from_lex(const d2 *__restrict__ lex, cx<double> *__restrict__ gauge,
                                                            int mu, Geom g, int in0, int in1, int in2, int in3) {
@@ -61,11 +62,11 @@ __global__ __launch_bounds__(kBlock) void k_gauge_from_lex(const d2 *__restrict_
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
         const d2 v = a[e];
-        dst[(size_t)e * g.stride + k] = {v.x, v.y};
+        dst[link_entry<double>(e, g.stride, k)] = {v.x, v.y};
     }
 }
 
-// Direction mu of the gauge planes -> 18 doubles per site, sites in lexicographic order
+// Direction mu of the gauge blocks -> 18 doubles per site, sites in lexicographic order
 // (lexicographic != 0, the file's order) or in HILA's EVEN_SITES_FIRST order.
 __global__ __launch_bounds__(kBlock) void k_gauge_to_host(const cx<double> *__restrict__ gauge, d2 *__restrict__ out,
                                                           int mu, Geom g, int lexicographic) {
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_gauge_to_host(const cx<double> *__re
     d2 *o = out + (size_t)9 * i;
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
-        const cx<double> v = src[(size_t)e * g.stride + k];
+        const cx<double> v = src[link_entry<double>(e, g.stride, k)];
         o[e] = {v.x, v.y};
     }
 }

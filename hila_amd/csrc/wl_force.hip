@@ -29,8 +29,8 @@ enum { FORCE_OUT = 0, FORCE_MOMENTUM = 1 };
 struct ForceArgs {
     const cx<double> *chi, *psi;  // even fields
     const cx<double> *tA, *tB;    // odd fields
-    const cx<double> *gauge;      // gauge planes (momentum mode)
-    cx<double> *out;              // force planes, or momentum planes (read-modify-write)
+    const cx<double> *gauge;      // gauge blocks (momentum mode)
+    cx<double> *out;              // force blocks, or momentum blocks (read-modify-write), site-tiled as the links
     double kappa;                 // F carries kappa
     double coef;                  // momentum mode: mom += P(U coef F)
     Geom g;
@@ -106,16 +106,16 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs A) {
         case 2: link_force<2, S>(F, A, p, k, kn, bc); break;
         default: link_force<3, S>(F, A, p, k, kn, bc); break;
     }
-    const size_t base = (size_t)((p * 4 + mu) * 9) * S_ + k;
+    const size_t base = (size_t)((p * 4 + mu) * 9) * S_;
     if constexpr (MODE == FORCE_OUT) {
 #pragma unroll
-        for (int e = 0; e < 9; ++e) stc<true>(A.out + base + (size_t)e * S_, A.kappa * F[e / 3][e % 3]);
+        for (int e = 0; e < 9; ++e) stc<true>(A.out + base + link_entry<double>(e, S_, k), A.kappa * F[e / 3][e % 3]);
     } else {
         // M = U (coef kappa F); mom += (M - M^dag)/2 - tr((M - M^dag)/2)/3
         const double c = A.coef * A.kappa;
         cx<double> U[3][3], M[3][3];
 #pragma unroll
-        for (int e = 0; e < 9; ++e) U[e / 3][e % 3] = ldc<true>(A.gauge + base + (size_t)e * S_);
+        for (int e = 0; e < 9; ++e) U[e / 3][e % 3] = ldc<true>(A.gauge + base + link_entry<double>(e, S_, k));
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs A) {
                 // ((M - M^dag)/2)_ij = ((M_ij.x - M_ji.x)/2, (M_ij.y + M_ji.y)/2)
                 cx<double> ah = {0.5 * (M[i][j].x - M[j][i].x), 0.5 * (M[i][j].y + M[j][i].y)};
                 if (i == j) ah.y -= tr;
-                cx<double> *q = A.out + base + (size_t)(3 * i + j) * S_;
+                cx<double> *q = A.out + base + link_entry<double>(3 * i + j, S_, k);
                 stc<false>(q, ldc<false>(q) + ah);
             }
     }
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kBlock) void k_sub(cx<double> *__restrict__ y, cons
     if (k >= vh) return;
 #pragma unroll
     for (int c = 0; c < 12; ++c) {
-        const size_t e = (size_t)c * S + k;
+        const size_t e = comp_elem<double>(c, S, k);
         y[e] = y[e] - x[e];
     }
 }

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "wl_host.h"
+#include "wl_store.h"
 
 namespace wl {
 
@@ -56,7 +57,7 @@ __global__ __launch_bounds__(kBlock) void k_mre_dots(const cx<double> *__restric
     for (int k = blockIdx.x * kBlock + threadIdx.x; k < vh; k += gridDim.x * kBlock) {
 #pragma unroll 1
         for (int c = 0; c < 12; ++c) {
-            const size_t e = (size_t)c * S + k;
+            const size_t e = comp_elem<double>(c, S, k);
             cx<double> b[NB], ab[NB];
 #pragma unroll
             for (int i = 0; i < NB; ++i) {
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_mre_combine(cx<double> *__restrict__
     const size_t rs = (size_t)12 * S;
 #pragma unroll
     for (int c = 0; c < 12; ++c) {
-        const size_t e = (size_t)c * S + k;
+        const size_t e = comp_elem<double>(c, S, k);
         cx<double> s = {0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < NB; ++i) s = s + cf.c[i] * ldc<true>(B + i * rs + e);

@@ -41,10 +41,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_mr(const cx<double> *__restr
     const int k = blockIdx.x * kBlock + threadIdx.x, j = blockIdx.y;
     double s = 0;
     if (k < vh) {
-        const size_t off = (size_t)j * 12 * S + k;
+        const size_t off = (size_t)j * 12 * S;
 #pragma unroll
         for (int c = 0; c < 12; ++c) {
-            const size_t e = off + c * S;
+            const size_t e = off + comp_elem<double>(c, S, k);
             const cx<double> v = b[e] - ddp[e];
             r[e] = v;
             p[e] = v;
@@ -68,13 +68,13 @@ __global__ __launch_bounds__(kBlock) void k_residual_update_mr(
     const bool on = first || st[j].done == 3 || st[j].done == 4;  // block-uniform
     double s = 0;
     if (on && k < vh) {
-        const size_t off = (size_t)j * 12 * S + k;
+        const size_t off = (size_t)j * 12 * S;
         cx<R> v[4][3];
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const size_t e = off + (size_t)(a * 3 + c) * S;
+                const size_t e = off + comp_elem<double>(a * 3 + c, S, k);
                 const cx<double> d = b[e] - ddp[e];
                 v[a][c] = {(R)d.x, (R)d.y};
                 s += norm2(d);
@@ -114,10 +114,10 @@ __global__ __launch_bounds__(kBlock) void k_fold_mr(cx<double> *__restrict__ x, 
     if (gate && *gate) return;
     const int k = blockIdx.x * kBlock + threadIdx.x, j = blockIdx.y;
     if (k >= vh || !(all || st[j].done == 3 || st[j].done == 4)) return;
-    const size_t off = (size_t)j * 12 * S + k;
+    const size_t off = (size_t)j * 12 * S;
 #pragma unroll
     for (int c = 0; c < 12; ++c) {
-        const size_t e = off + (size_t)c * S;
+        const size_t e = off + comp_elem<double>(c, S, k);
         const cx<float> v = y[e];
         x[e] = x[e] + cx<double>{(double)v.x, (double)v.y};
         y[e] = {0.f, 0.f};
@@ -139,9 +139,9 @@ __global__ __launch_bounds__(kBlock) void k_zero_rhs(cx<double> *__restrict__ x,
                                                      int S) {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= vh) return;
-    const size_t off = (size_t)j * 12 * S + k;
+    const size_t off = (size_t)j * 12 * S;
 #pragma unroll
-    for (int c = 0; c < 12; ++c) x[off + c * S] = {0.0, 0.0};
+    for (int c = 0; c < 12; ++c) x[off + comp_elem<double>(c, S, k)] = {0.0, 0.0};
 }
 
 // Finish of a per-rhs reduction: block j sums partials[j * pstride + 0 .. n) in order and

@@ -260,12 +260,12 @@ __device__ __forceinline__ void rb_epilogue(const RbArgs<T> &A, cx<typename St<T
             scale_unless_one(ddp, A.b);
             ld_site<T, true>(A.p + eb, S, k, pv);
             ld_site<T, true>(A.r + eb, S, k, rv);
-            const size_t xb = (size_t)(jf + r) * 12 * S + k;
+            const size_t xb = (size_t)(jf + r) * 12 * S;
 #pragma unroll
             for (int s = 0; s < 4; ++s)
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const size_t e = xb + (size_t)(s * 3 + c) * S;
+                    const size_t e = xb + comp_elem<double>(s * 3 + c, S, k);
                     stc<true>(A.x + e, ldc<true>(A.x + e) + alpha_r * pv[s][c]);
                     const cx<R> dd = ddp[s][c] + a * acc[r][s][c];
                     rv[s][c] = rv[s][c] - alpha_r * dd;

@@ -8,12 +8,15 @@
 //           |re|, |im| of the site, so each stored half lies in [-1, 1] and the relative error
 //           of a component is at most 2^-11 of the site's largest one.  Links are stored
 //           without scale (SU(3) entries are bounded by 1).
-// A right-hand side of an N-rhs field is a block of `planes` x `stride` elements.  Its 12
-// spin-colour components are packed P to a 16-byte vector (P = 1 FP64, 2 FP32, 4 FP16), so
-// that a lane reads a site's spinor with 12 / P vector loads: component c of site k is element
-// (c / P) * PS + Pk + c % P.  f16s adds the plane of FP32 site scales at element 12S (13S
-// elements of 4 bytes in all).  Links likewise: entries 0-7 of site k packed, at
-// (e / P) * PS + Pk + e % P, entry 8 at 8S + k, in a (parity, direction) block of 9S.
+// A right-hand side of an N-rhs field is a block of `planes` x `stride` elements (stride S, a
+// multiple of 64 sites).  Its 12 spin-colour components are packed P to a 16-byte vector
+// (P = 1 FP64, 2 FP32, 4 FP16), so that a lane reads a site's spinor with 12 / P vector loads,
+// and stored in site tiles: the sites are cut into tiles of 64, and tile t holds, one after
+// the other, vector 0 of its 64 sites, then vector 1, and so on (tile_elem).  A wave's 64
+// sites thus read and write one contiguous 12 x 64 x 16 = 12 KB stretch (FP64) instead of 12
+// pieces a plane apart.  f16s adds the plane of FP32 site scales at element 12S (13S elements
+// of 4 bytes in all).  Links likewise, in a (parity, direction) block of 9S: entries 0-7 of a
+// site packed P to a vector, and (for P > 1) entry 8, not packed, after them in the tile.
 #pragma once
 
 #include "wl_device.h"
@@ -123,16 +126,31 @@ __device__ __forceinline__ void st_vec(typename St<T>::E *v, const Packed<typena
         *p = bits;
 }
 
-// Element of spinor component c of site k in an rhs block; of link entry e in a link block.
+// Sites per tile of the site-tiled layout: one wave's sites.
+constexpr int kTile = 64;
+
+// Element of entry e of site k in a block of ne elements per site, packed P to a vector
+// (entries e < ne - ne % P), the rest one per site after the packed vectors of the tile.
+template <int P>
+__host__ __device__ __forceinline__ size_t tile_elem(int e, int ne, int k) {
+    const size_t tile = (size_t)(k / kTile) * ne * kTile;
+    const int lane = k % kTile;
+    const int npk = ne - ne % P;
+    if (e < npk) return tile + (size_t)(e / P) * P * kTile + (size_t)P * lane + (e % P);
+    return tile + (size_t)e * kTile + lane;
+}
+
+// Element of spinor component c of site k in an rhs block; of link entry e in a link block of
+// LF entries per site (9, or 6 for two-row links).  S, the block stride, only sizes the block.
 template <typename T>
 __device__ __forceinline__ size_t comp_elem(int c, int S, int k) {
-    constexpr int P = St<T>::pack;
-    return (size_t)(c / P) * P * S + (size_t)P * k + (c % P);
+    (void)S;
+    return tile_elem<St<T>::pack>(c, 12, k);
 }
-template <typename T>
+template <typename T, int LF = 9>
 __device__ __forceinline__ size_t link_entry(int e, int S, int k) {
-    constexpr int P = St<T>::pack;
-    return e < 8 ? (size_t)(e / P) * P * S + (size_t)P * k + (e % P) : (size_t)8 * S + k;
+    (void)S;
+    return tile_elem<St<T>::pack>(e, LF, k);
 }
 
 // Raw (packed) loads of a site's 12 spinor components and 9 link entries.
@@ -142,7 +160,7 @@ __device__ __forceinline__ void ld_spinor_raw(typename St<T>::E (&w)[12], const 
     constexpr int P = St<T>::pack;
 #pragma unroll
     for (int q = 0; q < 12 / P; ++q) {
-        const auto v = ld_vec<NT, T>(base + (size_t)q * P * S + (size_t)P * k);
+        const auto v = ld_vec<NT, T>(base + tile_elem<P>(P * q, 12, k));
 #pragma unroll
         for (int i = 0; i < P; ++i) w[P * q + i] = v.e[i];
     }
@@ -161,9 +179,9 @@ __device__ __forceinline__ void third_row(cx<R> (&u)[9]) {
 }
 
 // The 9 entries of the link at site k of a link block.  LF = 9: all entries stored (block of
-// 9S elements).  LF = 6: rows 0 and 1 stored (block of 6S elements, entry e < 6 at the same
-// place as in the full block), row 2 rebuilt by third_row: two-row storage, for links the
-// host has checked to be special unitary to rounding (FP64 and FP32 storage).
+// 9S elements).  LF = 6: rows 0 and 1 stored (block of 6S elements, tiles of 6 entries per
+// site), row 2 rebuilt by third_row: two-row storage, for links the host has checked to be
+// special unitary to rounding (FP64 and FP32 storage).
 template <bool NT, typename T, int LF = 9>
 __device__ __forceinline__ void ld_link_raw(typename St<T>::E (&u)[9], const typename St<T>::E *base, int S,
                                             int k) {
@@ -171,12 +189,12 @@ __device__ __forceinline__ void ld_link_raw(typename St<T>::E (&u)[9], const typ
     static_assert(LF == 9 || (LF == 6 && !St<T>::scaled), "two-row links: FP64 and FP32 storage");
 #pragma unroll
     for (int q = 0; q < (LF == 9 ? 8 : 6) / P; ++q) {
-        const auto v = ld_vec<NT, T>(base + (size_t)q * P * S + (size_t)P * k);
+        const auto v = ld_vec<NT, T>(base + tile_elem<P>(P * q, LF, k));
 #pragma unroll
         for (int i = 0; i < P; ++i) u[P * q + i] = v.e[i];
     }
     if constexpr (LF == 9)
-        u[8] = ldraw<NT>(base + (size_t)8 * S + k);
+        u[8] = ldraw<NT>(base + tile_elem<P>(8, 9, k));
     else
         third_row(u);
 }
@@ -220,7 +238,7 @@ __device__ __forceinline__ void st_site(typename St<T>::E *base, int S, int k,
             Packed<E, P> x;
 #pragma unroll
             for (int i = 0; i < P; ++i) x.e[i] = v[(P * q + i) / 3][(P * q + i) % 3];
-            st_vec<NT, T>(base + (size_t)q * P * S + (size_t)P * k, x);
+            st_vec<NT, T>(base + tile_elem<P>(P * q, 12, k), x);
         }
     } else {
         float m = 0.f;
@@ -241,7 +259,7 @@ __device__ __forceinline__ void st_site(typename St<T>::E *base, int S, int k,
                 x.e[i] = h;
                 v[c / 3][c % 3] = {m * (float)h.x, m * (float)h.y};
             }
-            st_vec<NT, T>(base + (size_t)q * P * S + (size_t)P * k, x);
+            st_vec<NT, T>(base + tile_elem<P>(P * q, 12, k), x);
         }
         if constexpr (NT)
             __builtin_nontemporal_store(m, scale_plane(base, S) + k);

@@ -55,6 +55,36 @@ def test_multi_rhs_operator_equals_single_rhs(N, prec):
                 assert rel(fo.to_full(j), ref) <= 1e-13
 
 
+@pytest.mark.parametrize("dims", [(6, 10, 14, 12), (24, 24, 24, 48)], ids=["6x10x14x12", "24x24x24x48"])
+def test_multi_rhs_operator_on_lattices_not_powers_of_two(dims):
+    """The multi-rhs kernel finds a site's coordinates by multiply-high division by the x, y
+    and z sizes; sizes 6, 10, 14 and 24 (not powers of two) check that division against the
+    oracle's neighbours, at every rhs-per-lane setting.  6x10x14x12 also ends in a partly
+    filled wave (2520 sites per parity); 24^3x48 runs the (z,t)-chunked traversal."""
+    lat, U = _setup(dims)
+    D = W.Dirac_Wilson_evenodd(0.125, lat)
+    orc = Oracle(dims)
+    N = 3
+    srcs = [W.gen_spinor(dims, 600 + j, EVEN) for j in range(N)]
+    refs = {sign: [orc.hop_set(U, 0.125, s, ODD, sign) for s in srcs] for sign in (1, -1)}
+    for prec, per_lane, tol in ((W.F64, (1, 2), 1e-13), (W.F32, (1, 2, 4), 2e-6), (W.F16, (1, 2, 4), 2e-3)):
+        fin = W.Field(lat, EVEN, prec, nrhs=N)
+        for j, s in enumerate(srcs):
+            fin.upload(s, rhs=j)
+        fo = W.Field(lat, ODD, prec, nrhs=N)
+        for nr in per_lane:
+            lat.set_option(W.OPT_RHS_BLOCK, nr)
+            for sign in (1, -1):
+                print(f"multi-rhs {dims} prec {prec} per lane {nr} sign {sign}", flush=True)
+                D.hop_set(fin, fo, sign)
+                for j in range(N):
+                    assert rel(fo.to_full(j), refs[sign][j]) <= tol, (prec, nr, sign, j)
+        lat.set_option(W.OPT_RHS_BLOCK, 0)
+        fin.close()
+        fo.close()
+    lat.close()
+
+
 def test_rhs_block_and_workgroup_options_do_not_change_results():
     dims, N = (16, 16, 16, 16), 12
     lat, U = _setup(dims)
