@@ -431,12 +431,11 @@ int gauge_upload_async(wl_ctx *c, const double *const U[4]) {
 
 namespace {
 
-// Default plane padding, from the single-rhs hopping term measured on MI355X (FP64 and FP32,
-// interleaved timings): 128 sites when the unpadded plane is a multiple of 2^17 sites (lattices
-// such as 32^4, whose FP64 planes are then 8 MiB apart), which made the hop 2-3% faster (32^4
-// FP64 142.9 -> 138.9 us); none otherwise (48^3x96, 2^16 x 81 sites per plane, and 24^3x48 were
-// 0.5-1.5% slower with any pad from 64 to 960 sites).
-int default_plane_pad(int base) { return base % (1 << 17) == 0 ? 128 : 0; }
+// Default block padding: none.  With the site-tiled layout the hopping term (FP64 and FP32
+// single-rhs, FP64 and FP16 at 12 rhs; 32^4, 24^3x48, 48^3x96; interleaved timings on MI355X)
+// times within 1% for every pad from 0 to 256 sites.  (With one plane per component, 128 sites
+// had made the 32^4 hop 2-3% faster, by moving planes off 8 MiB distances.)
+constexpr int kDefaultPlanePad = 0;
 
 // Segments (64 checkerboard sites) per time slice, or 0 when a slice is not a whole number of
 // segments (then only the memory-order traversal is available).
@@ -459,12 +458,11 @@ int auto_chunk(const Geom &g) {
 }
 
 // Block stride S in sites (a field's rhs block holds planes x S elements, site-tiled): the
-// parity's sites rounded up to a workgroup, plus `pad` sites (pad < 0: the default,
-// default_plane_pad).  Always a multiple of the 64-site tile.
+// parity's sites rounded up to a workgroup, plus `pad` sites (pad < 0: kDefaultPlanePad).
+// Always a multiple of the 64-site tile.
 int plane_stride(int vh, int pad) {
     const int base = (vh + kBlock - 1) / kBlock * kBlock;
-    if (pad < 0) pad = default_plane_pad(base);
-    return base + pad;
+    return base + (pad < 0 ? kDefaultPlanePad : pad);
 }
 
 int set_chunk(Geom &g, int qc) {

@@ -302,11 +302,10 @@ int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, d
  *   WL_OPT_RHS_LAUNCH  right-hand sides per launch, i.e. per pass over the lattice (rounded
  *       to a multiple of the per-lane count; 0 = default).  Fewer per pass keep the neighbour
  *       spinors of a pass in cache; each pass reads the links again.
- *   WL_OPT_PLANE_PAD  sites of padding added to every component plane of the device layout
- *       (a multiple of 64, 0..65536; -1 = default: 128 when the unpadded plane is a multiple
- *       of 2^17 sites, as at 32^4, else 0), so that planes do not sit at large power-of-two
- *       distances.  Only before the gauge field or any field of the context exists (else
- *       WL_E_STATE).
+ *   WL_OPT_PLANE_PAD  sites of padding added to the block stride of the device layout (a
+ *       multiple of 64, 0..65536; -1 = default: none).  Moves addresses only; with the
+ *       site-tiled layout it measured neutral.  Only before the gauge field or any field of the
+ *       context exists (else WL_E_STATE).
  *   WL_OPT_LINK_LOAD  link loads of the multi-rhs kernel: 0 = automatic (nontemporal when a
  *       launch reads each link once), 1 = cached, 2 = nontemporal, 3 = staged in LDS: one
  *       workgroup per 64-site segment whose waves (at most 8) are the rhs chunks share one read
