@@ -431,11 +431,15 @@ int gauge_upload_async(wl_ctx *c, const double *const U[4]) {
 
 namespace {
 
-// Default block padding: none.  With the site-tiled layout the hopping term (FP64 and FP32
-// single-rhs, FP64 and FP16 at 12 rhs; 32^4, 24^3x48, 48^3x96; interleaved timings on MI355X)
-// times within 1% for every pad from 0 to 256 sites.  (With one plane per component, 128 sites
-// had made the 32^4 hop 2-3% faster, by moving planes off 8 MiB distances.)
-constexpr int kDefaultPlanePad = 0;
+// Default block padding: 128 sites when the unpadded stride is a multiple of 2^17 sites
+// (lattices such as 32^4), else none.  Unpadded, the rhs blocks of a 32^4 FP64 field sit
+// 96 MiB = 3 x 2^25 B apart, so kernels that read several rhs at one site at once (multi-rhs
+// hop, MRE combine) send those streams to the same DRAM channels.  Measured on MI355X with the
+// site-tiled layout, interleaved: the 12-rhs hop 0.7-1.3% faster with the pad (FP64 81.4 ->
+// 80.8 us, FP16 21.45 -> 21.18 us per rhs), the single-rhs hop within 0.3%; the 8-vector
+// MRE combine 206 us unpadded against 161 us padded.  Other lattices (24^3x48, 48^3x96) timed
+// within 1% for every pad from 0 to 256 sites.
+int default_plane_pad(int base) { return base % (1 << 17) == 0 ? 128 : 0; }
 
 // Segments (64 checkerboard sites) per time slice, or 0 when a slice is not a whole number of
 // segments (then only the memory-order traversal is available).
@@ -458,11 +462,11 @@ int auto_chunk(const Geom &g) {
 }
 
 // Block stride S in sites (a field's rhs block holds planes x S elements, site-tiled): the
-// parity's sites rounded up to a workgroup, plus `pad` sites (pad < 0: kDefaultPlanePad).
+// parity's sites rounded up to a workgroup, plus `pad` sites (pad < 0: default_plane_pad).
 // Always a multiple of the 64-site tile.
 int plane_stride(int vh, int pad) {
     const int base = (vh + kBlock - 1) / kBlock * kBlock;
-    return base + (pad < 0 ? kDefaultPlanePad : pad);
+    return base + (pad < 0 ? default_plane_pad(base) : pad);
 }
 
 int set_chunk(Geom &g, int qc) {

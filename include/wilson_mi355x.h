@@ -303,9 +303,10 @@ int wl_hasenbusch2_force_step(wl_ctx *ctx, const wl_field *chi, wl_field *psi, d
  *       to a multiple of the per-lane count; 0 = default).  Fewer per pass keep the neighbour
  *       spinors of a pass in cache; each pass reads the links again.
  *   WL_OPT_PLANE_PAD  sites of padding added to the block stride of the device layout (a
- *       multiple of 64, 0..65536; -1 = default: none).  Moves addresses only; with the
- *       site-tiled layout it measured neutral.  Only before the gauge field or any field of the
- *       context exists (else WL_E_STATE).
+ *       multiple of 64, 0..65536; -1 = default: 128 when the unpadded stride is a multiple of
+ *       2^17 sites, as at 32^4, else 0), so that the rhs blocks of a field do not sit at large
+ *       power-of-two distances.  Moves addresses only.  Only before the gauge field or any
+ *       field of the context exists (else WL_E_STATE).
  *   WL_OPT_LINK_LOAD  link loads of the multi-rhs kernel: 0 = automatic (nontemporal when a
  *       launch reads each link once), 1 = cached, 2 = nontemporal, 3 = staged in LDS: one
  *       workgroup per 64-site segment whose waves (at most 8) are the rhs chunks share one read

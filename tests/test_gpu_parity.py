@@ -251,8 +251,9 @@ def test_traversal_and_workgroup_options_do_not_change_results(dims):
 def test_plane_padding_changes_no_bits():
     """Block padding (WL_OPT_PLANE_PAD) moves addresses only.  M in every storage format, with
     1 and 3 rhs, and the solutions of FP64 CG, batched FP16-inner CG and block CG are
-    bitwise identical with no padding and with 192 sites.  The default pads nothing."""
-    for dims, want in (((32, 32, 32, 32), 0), ((24, 24, 24, 48), 0), ((16, 16, 16, 16), 0)):
+    bitwise identical with no padding and with 192 sites.  The default pads 128 sites exactly
+    where the unpadded stride is a multiple of 2^17 sites."""
+    for dims, want in (((32, 32, 32, 32), 128), ((24, 24, 24, 48), 0), ((16, 16, 16, 16), 0)):
         lat = W.Lattice(dims)
         assert lat.get_option(W.OPT_PLANE_PAD) == want, dims
         lat.close()
