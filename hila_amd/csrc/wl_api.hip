@@ -433,12 +433,10 @@ namespace {
 
 // Default block padding: 128 sites when the unpadded stride is a multiple of 2^17 sites
 // (lattices such as 32^4), else none.  Unpadded, the rhs blocks of a 32^4 FP64 field sit
-// 96 MiB = 3 x 2^25 B apart, so kernels that read several rhs at one site at once (multi-rhs
-// hop, MRE combine) send those streams to the same DRAM channels.  Measured on MI355X with the
-// site-tiled layout, interleaved: the 12-rhs hop 0.7-1.3% faster with the pad (FP64 81.4 ->
-// 80.8 us, FP16 21.45 -> 21.18 us per rhs), the single-rhs hop within 0.3%; the 8-vector
-// MRE combine 206 us unpadded against 161 us padded.  Other lattices (24^3x48, 48^3x96) timed
-// within 1% for every pad from 0 to 256 sites.
+// 96 MiB = 3 x 2^25 B apart.  Measured on MI355X with the site-tiled layout, interleaved: the
+// 12-rhs hop 0.7-1.3% faster with the pad (FP64 81.4 -> 80.8 us, FP16 21.45 -> 21.18 us per
+// rhs), the single-rhs hop within 0.3%.  Other lattices (24^3x48, 48^3x96) timed within 1% for
+// every pad from 0 to 256 sites.
 int default_plane_pad(int base) { return base % (1 << 17) == 0 ? 128 : 0; }
 
 // Segments (64 checkerboard sites) per time slice, or 0 when a slice is not a whole number of
