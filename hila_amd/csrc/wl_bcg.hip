@@ -41,7 +41,7 @@ namespace {
 constexpr int kBcgMax = 24;    // right-hand sides of one block
 
 // The FP32 inner solution of the mixed solvers (BatchCG's y, folded into x by k_fold_mr) is
-// laid out as the FP64 solution is, one complex number per component plane: unpacked FP32.
+// laid out as the FP64 solution is, one complex number per component: unpacked FP32.
 struct f32u {};
 constexpr int kF32Unpacked = 100;  // its precision code in launch_bcombo
 constexpr int kGramWg = 1024;  // workgroups of a Gram reduction (partials)

@@ -9,7 +9,7 @@
 //
 // The file is streamed through two pinned chunk buffers into the context's device staging
 // buffer (one direction at a time), and one kernel per direction scatters the lexicographic
-// links into the gauge planes, tiling a smaller file lattice as it goes; writing runs the
+// links into the gauge blocks, tiling a smaller file lattice as it goes; writing runs the
 // same pipeline backwards.  Host code only reads and writes bytes: every reordering is done
 // on the GPU.
 

@@ -250,10 +250,10 @@ __device__ __forceinline__ void su3_adj_mul(const cx<R> (&u)[3][3], const cx<R> 
 struct Geom {
     int L[4];        // Lx, Ly, Lz, Lt (all even)
     int vh;          // sites per parity
-    int stride;      // elements between two component planes of a field (>= vh, padded)
+    int stride;      // block stride S in sites: an rhs block holds 12 x S elements (>= vh, padded)
     int antiperiodic;
     // Traversal of the hopping kernel: a wave owns one 64-site segment of the checkerboard
-    // (64 consecutive k: coalesced plane accesses).  qc = 0 walks segments in memory order;
+    // (64 consecutive k: one site tile, coalesced accesses).  qc = 0 walks segments in memory order;
     // qc > 0 walks them chunk by chunk, each chunk = qc consecutive segments of a time slice,
     // taken for every t before the next chunk, so that the t neighbours of a site are read
     // while their spinors are still in the XCD's L2.  sps = segments per time slice,

@@ -2,7 +2,7 @@
 //
 // For output site x (parity p) it computes H_s in(x) = sum_mu [ U_mu(x)(1 + s g_mu) in(x+mu)
 // + U_mu(x-mu)^dag (1 - s g_mu) in(x-mu) ] in one pass: each neighbour spinor is gathered from
-// its component planes (coalesced 16-byte loads in FP64), spin-projected in registers to a
+// its site tile (coalesced 16-byte loads in FP64), spin-projected in registers to a
 // half spinor, multiplied by the link (or its adjoint) and reconstructed into the accumulator.
 // This replaces HILA's 8 projection kernels, 8 half-vector temporaries and 4 read-modify-write
 // accumulation kernels per half operator (libraries/dirac/wilson.h:15-95) with one kernel that

@@ -193,7 +193,7 @@ namespace wl {
 
 namespace {
 
-// Link planes (gauge layout, FP64) held by the context, zero when first created.
+// Link blocks (gauge layout, FP64) held by the context, zero when first created.
 int ensure_links(wl_ctx *c, void **p, const char *what) {
     if (*p) return WL_OK;
     const size_t bytes = gauge_elems(c->g) * 16;

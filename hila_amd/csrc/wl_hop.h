@@ -1,7 +1,7 @@
 // The pipelined Wilson hopping sum shared by the single- and multi-rhs kernels.
 //
 // acc = sum_mu [ U_mu(x)(1 + s g_mu) in(x+mu) + U_mu(x-mu)^dag (1 - s g_mu) in(x-mu) ] for one
-// output site: each neighbour spinor is gathered from its component planes, spin-projected in
+// output site: each neighbour spinor is gathered from its site tile, spin-projected in
 // registers to a half spinor, multiplied by the link (or its adjoint) and reconstructed into
 // the accumulator (wilson.h:15-95 in one pass; HalfWilsonVector projection/expand of
 // wilson_vector.h:439-567).  The 8 hops are software-pipelined: hop h+1's link and spinor are
@@ -64,8 +64,8 @@ __device__ __forceinline__ void hop_compute(cx<R> (&acc)[4][3], const HopBuf<R> 
 
 // Hop H = 2 mu + (0 forward, 1 backward): issue hop H+1's loads, then compute hop H.
 // nsp: neighbour element offsets of the spinor field (site index times the rhs count);
-// nl: neighbour site indices (backward links live at the neighbour); S: link plane stride;
-// Ssp: spinor plane stride; LF: stored entries per link (9, or 6 for two-row links).
+// nl: neighbour site indices (backward links live at the neighbour); S: link block stride;
+// Ssp: spinor block stride; LF: stored entries per link (9, or 6 for two-row links).
 template <typename R, int SIGN, int H, bool NTL = true, int LF = 9>
 __device__ __forceinline__ void hop_chain(cx<R> (&acc)[4][3], HopBuf<R> (&buf)[2],
                                           const cx<R> *__restrict__ in, const cx<R> *gout,

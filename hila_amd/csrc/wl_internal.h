@@ -16,10 +16,11 @@
 
 namespace wl {
 
-// Device layout of a Wilson-vector field of one parity: for every rhs, 12 component planes
-// (spin-major, colour-minor) of `stride` complex numbers; site k of the parity is element k of
-// each plane.  Gauge field: for parity p, direction mu, matrix entry e = 3*row + col, a plane of
-// `stride` complex numbers: gauge[((p*4 + mu)*9 + e)*stride + k] = U_mu(site k of parity p).
+// Device layout of a Wilson-vector field of one parity: for every rhs, a block of 12 x stride
+// complex numbers (spin-major, colour-minor components), site-tiled: component c of site k at
+// comp_elem(c, stride, k) (wl_store.h).  Gauge field: for parity p and direction mu a block of
+// 9 x stride complex numbers starting at ((p*4 + mu)*9)*stride, entry e = 3*row + col of the
+// link U_mu(site k of parity p) at link_entry(e, stride, k) in it.
 inline size_t field_elems(const Geom &g, int nrhs) { return (size_t)nrhs * 12 * g.stride; }
 inline size_t gauge_elems(const Geom &g) { return (size_t)2 * 4 * 9 * g.stride; }
 
@@ -74,7 +75,7 @@ int two_rows_blocks(const Geom &g);
 inline size_t rows_elems(const Geom &g) { return (size_t)2 * 4 * 6 * g.stride; }
 hipError_t launch_gauge_convert(const Geom &g, int prec, const void *gauge64, void *gauge_lo,
                                 hipStream_t s);
-// direction mu of link planes (gauge layout, FP64) -> 18 doubles per site, sites in
+// direction mu of link blocks (gauge layout, FP64) -> 18 doubles per site, sites in
 // lexicographic order (lexicographic != 0) or EVEN_SITES_FIRST order (wl_config.hip)
 hipError_t launch_links_to_host(const Geom &g, const void *planes, int mu, double *out, int lexicographic,
                                 hipStream_t s);

@@ -21,7 +21,7 @@ namespace wl {
 
 namespace {
 
-constexpr int kNaMaxStride = 1 << 15;                          // complex numbers per plane
+constexpr int kNaMaxStride = 1 << 15;                          // block stride, sites
 constexpr size_t kNaField = (size_t)12 * kNaMaxStride * 16;    // one FP64 field, one rhs
 constexpr size_t kNaPartials = 1024;                           // doubles
 constexpr size_t kNaSmall = 256;                               // CG state, barrier, reduction out
@@ -68,8 +68,8 @@ extern "C" int wl_cg_solve_noalloc(wl_ctx *c, wl_field *x, const wl_field *b, do
     if (!std::isfinite(kappa)) return fail(WL_E_ARG, "kappa must be finite");
     if (c->g.stride > kNaMaxStride)
         return fail(WL_E_ARG,
-                    "wl_cg_solve_noalloc: %d sites per parity (plane stride %d) exceed the fixed workspace "
-                    "(plane stride at most %d, e.g. 16^4)",
+                    "wl_cg_solve_noalloc: %d sites per parity (block stride %d) exceed the fixed workspace "
+                    "(block stride at most %d, e.g. 16^4)",
                     c->g.vh, c->g.stride, kNaMaxStride);
     if (cg_partials_needed(c) > kNaPartials)
         return fail(WL_E_ARG, "wl_cg_solve_noalloc: %zu partial sums exceed the fixed workspace's %zu",

@@ -78,7 +78,7 @@ int slot_capacity(int n) {
     return n;
 }
 
-// Device bytes of a slot (fields at plane stride S, gauge copies, staging).  link_rows: the
+// Device bytes of a slot (fields at block stride S, gauge copies, staging).  link_rows: the
 // slots' WL_OPT_LINK_ROWS.  With two-row links BatchCG::prepare makes the FP64 two-row copy
 // for the reliable updates at every inner precision, and the FP32 inner operator reads an
 // FP32 two-row copy instead of the full FP32 links; FP16 always reads full FP16 links.
