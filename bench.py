@@ -865,7 +865,11 @@ def main(argv=None):
                     help="short run for rocprofv3: --steps hopping launches, then --steps "
                          "plain field copies (12 planes of V/2 complex doubles each way) as "
                          "the byte-counter calibration; prints no JSON line")
+    ap.add_argument("--lattice", type=int, nargs=4, default=None, metavar=("LX", "LY", "LZ", "LT"),
+                    help="with --profile: the lattice of the profiled launches (default 32 32 32 32)")
     a = ap.parse_args(argv)
+    if a.lattice is not None and not a.profile:
+        ap.error("--lattice applies to --profile only; the benchmark runs its fixed workloads")
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is a design note only)"}))
@@ -893,9 +897,10 @@ def main(argv=None):
         return
 
     t_setup = time.perf_counter()
-    U = W.gen_gauge(DIMS, "smooth", 0.3, 3)
-    psi = W.gen_spinor(DIMS, 4, W.EVEN)
-    lat = W.Lattice(DIMS, antiperiodic_t=True)
+    dims = tuple(a.lattice) if a.lattice is not None else DIMS
+    U = W.gen_gauge(dims, "smooth", 0.3, 3)
+    psi = W.gen_spinor(dims, 4, W.EVEN)
+    lat = W.Lattice(dims, antiperiodic_t=True)
     lat.upload_gauge(U)
     D = W.Dirac_Wilson_evenodd(KAPPA, lat)
     fin = W.Field(lat, W.EVEN).upload(psi)
